@@ -1,0 +1,173 @@
+"""Benchmark of the explicit step on one MI355X.
+
+Workload: 3D 128^3 periodic (mw 8 x ppw 16), X0 = (0.1,0.2,0.3,0.2), T = 873 K,
+kappa = 5e-16, elasticity on (c_ij 240/150/120 GPa, V_m 7.1e-6), synthetic
+regular-solution table (step 0.025).  One step = interpolation + kappa lap(X) +
+rank-1 elastic FFT pair + face-flux divergence + adaptive dt + update/clamp of
+all 2,097,152 cells.  Inputs resident in HBM before the timed region.
+
+Reported:
+  value        FP32 fused device step (FP64 mass / dt / k=0 mode), cell-updates/s
+  fp64_fused   same step, FP64 device-resident fused path
+  fp64_plain   the per-function drop-in FP64 path (one device call per reference
+               function, host dt from abs_max each step)
+  error        max / RMS |X_fp32 - X_fp64| after the timed steps (same init)
+  roofline     fused stencil kernel K1: algorithmic bytes (36 B/cell FP32:
+               X 16 + psi 4 read, dX/dt 16 written) / its HIP-event duration
+  cpu_baseline C/OpenMP FP64 restatement of the reference loop + scipy.fft
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s batched FP32 vs plain FP64 at <=1e-5 err; p50/p99 sim latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pb, seconds=15.0, threads=None):
+    """Time the C/OpenMP restatement of the reference loop on a bounded sample of the workload."""
+    from oracle.cpu_path import CPath
+    threads = threads or os.cpu_count()
+    cp = CPath(pb.data_grid, pb.cfg.resolution, pb.tree, pb.calphad_values, pb.dx, pb.kappa, pb.X0, pb.l_vec,
+               pb.Bhat, pb.ctld, pb.dt_cfl, pb.cfg.lower_limit, pb.cfg.upper_limit, threads=threads)
+    X = pb.X_init.copy()
+    dt = pb.dt0
+    dt, _ = cp.step(X, dt)  # warm-up step
+    n, t0 = 0, time.perf_counter()
+    while True:
+        dt, _ = cp.step(X, dt)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or n >= 200:
+            break
+    return {"value": pb.ncell * n / el, "unit": "cell-updates/s", "cores": cp.threads(), "kind": "port",
+            "sample": f"{n} steps of the 3D 128^3 FP64 step (C/OpenMP restatement, scipy.fft 1 worker) "
+                      f"after 1 warm-up step, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--variant", type=int, default=None)
+    ap.add_argument("--cx", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the explicit step of one simulation "
+              "and the batched engine both live on a single device in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from phasefield5d_amd import _lib
+    from phasefield5d_amd.engine import Engine
+    from phasefield5d_amd.plain import PlainStepper
+    from phasefield5d_amd.problem import build_problem
+    from phasefield5d_amd.workloads import workload_config
+
+    _lib.require_gpu()
+    cfg = workload_config("3d_128", seed=0)
+    pb = build_problem(cfg)
+    ncell = pb.ncell
+    log(f"workload 3D {pb.shape} ncell={ncell} ctld={pb.ctld:.4g} dt0={pb.dt0:.3g} dt_cfl={pb.dt_cfl:.3g}")
+
+    def run_engine(precision, steps, warmup):
+        eng = Engine(precision)
+        slot = eng.add_problem(pb)
+        if args.variant is not None or args.cx is not None:
+            eng.set_tiling(args.variant or 0, args.cx or 16)
+        eng.step(warmup)
+        eng.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.step(steps)
+        eng.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        return eng, slot, el
+
+    # FP32 fused (headline)
+    log("fp32 fused: warmup", args.warmup, "steps", args.steps)
+    e32, s32, el32 = run_engine("fp32", args.steps, args.warmup)
+    v32 = ncell * args.steps / el32
+    k1_ms = e32.time_kernel("k1", reps=20)
+    fft_ms = e32.time_kernel("fft", reps=10)
+    info = e32.launch_info()
+    log(f"fp32: {v32:.4g} cell-updates/s, {1e3 * el32 / args.steps:.3f} ms/step; K1 {k1_ms:.3f} ms, FFT {fft_ms:.3f} ms",
+        info)
+    st32 = e32.state(s32)
+
+    # FP64 fused, same number of steps from the same state (error reference)
+    log("fp64 fused")
+    e64, s64, el64 = run_engine("fp64", args.steps, args.warmup)
+    v64 = ncell * args.steps / el64
+    k1_64 = e64.time_kernel("k1", reps=10)
+    X32, X64 = e32.field(s32, "X"), e64.field(s64, "X")
+    diff = np.abs(X32 - X64)
+    st64 = e64.state(s64)
+    mass0 = pb.X_init.reshape(-1, 4).sum(axis=0)
+    drift32 = np.abs(st32["mass"] - mass0) / mass0
+    log(f"fp64 fused: {v64:.4g} cell-updates/s; max|dX| {diff.max():.3e} rms {np.sqrt((diff**2).mean()):.3e}")
+    del e64
+
+    # plain FP64 drop-in path
+    log("fp64 plain drop-in")
+    ps = PlainStepper(pb)
+    ps.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        ps.step()
+    torch.cuda.synchronize()
+    elp = time.perf_counter() - t0
+    vplain = ncell * args.plain_steps / elp
+    log(f"fp64 plain: {vplain:.4g} cell-updates/s")
+    del ps
+
+    k1_bytes = 36 * ncell
+    achieved = k1_bytes / (k1_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": v32, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el32 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3D 128^3 periodic explicit step, FP32 fused device path (FP64 mass, dt, k=0 mode), "
+                               "elasticity on, synthetic regular-solution table",
+                   "cells": ncell, "shape": list(pb.shape), "precision": "fp32", "elastic": True},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_stencil<float> (fused interpolation + kappa lap + mu + face flux + divergence)",
+                     "bytes_per_cell": 36, "avg_ms": k1_ms},
+        "fp64_fused": {"value": v64, "ms_per_step": 1e3 * el64 / args.steps, "k1_ms": k1_64},
+        "fp64_plain": {"value": vplain, "steps": args.plain_steps},
+        "ratio_fp32_vs_plain_fp64": v32 / vplain,
+        "error_vs_fp64": {"max": float(diff.max()), "rms": float(np.sqrt((diff ** 2).mean())),
+                          "steps": args.warmup + args.steps},
+        "mass_drift_fp32": drift32.tolist(),
+        "stage_ms": {"k1": k1_ms, "fft": fft_ms},
+        "launch": info,
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(pb, seconds=args.cpu_seconds)
+        log("cpu", out["cpu_baseline"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

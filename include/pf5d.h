@@ -1,0 +1,144 @@
+/* pf5d.h -- C ABI of the MI355X (gfx950) phasefield5d explicit-step library.
+ *
+ * All compute entry points take DEVICE pointers and a hipStream_t (passed as
+ * void*; NULL = the default stream) and return 0 on success or a negative
+ * PF5D_E* code; pf5d_last_error() gives the message of the last failure on the
+ * calling thread.  Arrays use the reference layout: C-order, the four
+ * independent components (Mn, Ni, Co, Cu) innermost.  `prec` selects the
+ * element type of device arrays: PF5D_F64 (double) or PF5D_F32 (float).
+ *
+ * Reference interfaces each entry point replaces are cited per declaration
+ * (paths relative to the phasefield5d repository).
+ */
+#ifndef PF5D_H
+#define PF5D_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF5D_F64 0
+#define PF5D_F32 1
+
+#define PF5D_OK 0
+#define PF5D_EINVAL (-1)   /* bad argument (maps to ValueError) */
+#define PF5D_EDEVICE (-2)  /* HIP runtime error (RuntimeError) */
+#define PF5D_ENOTIMPL (-3) /* unsupported dimension (NotImplementedError) */
+#define PF5D_ENOMEM (-4)   /* device allocation failed (MemoryError) */
+
+typedef struct pf5d_table pf5d_table;
+typedef struct pf5d_elastic pf5d_elastic;
+typedef struct pf5d_engine pf5d_engine;
+
+const char* pf5d_last_error(void);
+int pf5d_version(void);
+/* device properties used for launch sizing: [CU count, max LDS per block, warp size] */
+int pf5d_device_info(int out[3]);
+
+/* ---- CALPHAD table (solver/interpolation.py:13-35 build_4d_grid output) ----
+ * grid: host (num,num,num,num,nprops) float64 with NaN outside the simplex,
+ * nprops == 9.  The library keeps a padded device copy in `prec`. */
+int pf5d_table_create(int prec, int num, int nprops, const double* grid_host, double resolution,
+                      pf5d_table** out);
+int pf5d_table_destroy(pf5d_table* t);
+
+/* ---- per-function drop-in kernels (one reference function each) ---- */
+
+/* interpolator_nb -- solver/interpolation.py:154-211.  X (ncell,4) -> out rows of
+ * stride ostride (>= 9): linear where sum(X) <= 1-4h-1e-9, nearest node else. */
+int pf5d_interpolate(int prec, const void* X, long ncell, const pf5d_table* tab, void* out, long ostride,
+                     void* stream);
+
+/* calculate_laplacian -- solver/operators.py:10-81.  shape[dim], periodic. */
+int pf5d_laplacian(int prec, const void* X, int dim, const long* shape, double cell_size, void* out,
+                   void* stream);
+
+/* composition_laplacian *= kappa; mu -= composition_laplacian -- examples/simulate.py:254-255.
+ * mu rows of stride mstride (the interpolated-data view). */
+int pf5d_kappa_subtract(int prec, void* lap, void* mu, long mstride, long ncell, const double kappa[4],
+                        void* stream);
+
+/* get_divergence_from_mu_function(dim)(X, Mob, mu, inv_dx, out) -- solver/fluxes.py:296-416.
+ * mob / mu: rows of stride bstride / mstride (views into the interpolated data). */
+int pf5d_divergence_from_mu(int prec, const void* X, const void* mob, long bstride, const void* mu, long mstride,
+                            int dim, const long* shape, double inv_dx, void* out, void* stream);
+
+/* abs_max -- solver/operators.py:161-175.  Blocks until the value is on the host. */
+int pf5d_abs_max(int prec, const void* a, long n, double* host_result, void* stream);
+
+/* composition_change *= dt; current_composition += composition_change -- simulate.py:316-317 */
+int pf5d_update(int prec, void* X, void* dX, long nelem, double dt, void* stream);
+
+/* clamp_simplex -- solver/operators.py:262-317 */
+int pf5d_clamp_simplex(int prec, void* X, long ncell, void* stream);
+
+/* out_of_bounds statistics -- solver/system.py:12-53.  host_out5 =
+ * [nonfinite count, max sum, min sum, min component, max component]; blocks. */
+int pf5d_bounds_stats(int prec, const void* X, long ncell, double host_out5[5], void* stream);
+
+/* make_elastic_updater rank-1 path -- solver/elastic.py:147-263.  bhat_host:
+ * kernel on the rfft half grid (k_shape) in J/m^3; l, X0: rank-1 vector and
+ * reference composition.  apply(): mu[c*mstride + s] += psi[c] * l[s] with
+ * psi = irfftn(bhat * rfftn(sum_s (X_s - X0_s) l_s)). */
+int pf5d_elastic_create(int prec, int dim, const long* shape, const double* bhat_host, const double l[4],
+                        const double X0[4], pf5d_elastic** out);
+int pf5d_elastic_apply(pf5d_elastic* e, void* mu, long mstride, const void* X, void* stream);
+/* psi of the last apply() (device pointer, ncell elements of prec) */
+const void* pf5d_elastic_psi(const pf5d_elastic* e);
+int pf5d_elastic_destroy(pf5d_elastic* e);
+
+/* ---- device-resident fused step (examples/simulate.py:230-320 loop body) ---- */
+typedef struct pf5d_sim_desc {
+  int dim;            /* 1, 2, 3 */
+  long shape[3];      /* first dim entries used */
+  double cell_size;   /* h [m] */
+  double kappa[4];    /* kappa_i * kappa_value */
+  double X0[4];       /* reference composition (misfit origin) */
+  double l[4];        /* rank-1 elastic vector; ignored if elastic == 0 */
+  int elastic;        /* 1: bhat given */
+  double ctld, dt_cfl, dt0;
+  double lower_limit, upper_limit;
+  double t_end;       /* stop when t >= t_end (inf: never) */
+  int oob_interval;   /* bounds check period in steps (reference: 100; 0 = off) */
+  double tol_soft, tol_hard;
+} pf5d_sim_desc;
+
+/* stream: NULL -> the engine creates its own non-blocking stream */
+int pf5d_engine_create(int prec, void* stream, pf5d_engine** out);
+int pf5d_engine_destroy(pf5d_engine* e);
+/* X_host: (ncell,4) float64; bhat_host as for pf5d_elastic_create; returns slot */
+int pf5d_engine_add(pf5d_engine* e, const pf5d_sim_desc* d, const double* X_host, const double* bhat_host,
+                    const pf5d_table* tab, int* slot);
+int pf5d_engine_remove(pf5d_engine* e, int slot);
+/* K1 tile: variant index (pf5d_engine_num_variants), x chunk (3D), per dimension */
+int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* by_bz_nt /* 3*n ints, may be NULL */);
+int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx);
+/* enqueue nsteps (graph replay); does not block */
+int pf5d_engine_step(pf5d_engine* e, int nsteps);
+int pf5d_engine_sync(pf5d_engine* e);
+/* state[16] = dt, t, last_xmax, mass[4], steps, status, smax, smin, cmin, cmax, nonfinite, mean0, mean1 */
+int pf5d_engine_get_state(pf5d_engine* e, int slot, double* state16);
+int pf5d_engine_set_time(pf5d_engine* e, int slot, double dt, double t);
+/* copy fields to host as float64: which = 0 X, 1 dX/dt (of the last step), 2 psi */
+int pf5d_engine_get_field(pf5d_engine* e, int slot, int which, double* host_out);
+int pf5d_engine_set_field(pf5d_engine* e, int slot, const double* X_host);
+/* device pointer of a field (which as above) for zero-copy consumers */
+void* pf5d_engine_field_ptr(pf5d_engine* e, int slot, int which);
+void* pf5d_engine_stream(pf5d_engine* e);
+/* number of active slots and bytes of device memory held */
+int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes);
+/* last K1/K2/FFT launch configuration, for profiling reports: [k1 items, k2 items, fft items, k1 grid, k1 lds] */
+int pf5d_engine_launch_info(pf5d_engine* e, long out5[5]);
+
+/* average device time of one launch of a stage (HIP events on the engine stream):
+ * which = 0 fused stencil K1, 1 the FFT passes, 2 dt controller, 3 update kernel in
+ * its statistics-only mode.  The stencil is idempotent, so timing it leaves the
+ * state of the step unchanged. */
+int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PF5D_H */

@@ -1,0 +1,1261 @@
+// Host side of the phasefield5d MI355X step library: C ABI, FFT plans, the
+// device-resident engine (work lists over many simulations, graph replay).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pf5d.h"
+#include "pf5d_common.h"
+#include "pf5d_fft.h"
+#include "pf5d_flux.h"
+#include "pf5d_fused.h"
+#include "pf5d_interp.h"
+
+using namespace pf5d;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+struct Pf5dError : std::runtime_error {
+  int code;
+  Pf5dError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess)                                                                           \
+      throw Pf5dError(PF5D_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_) + " at line " + \
+                                        std::to_string(__LINE__));                                  \
+  } while (0)
+
+static void require(bool ok, const std::string& msg, int code = PF5D_EINVAL) {
+  if (!ok) throw Pf5dError(code, msg);
+}
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return PF5D_OK;
+  } catch (const Pf5dError& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return PF5D_ENOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return PF5D_EDEVICE;
+  }
+}
+
+static void* dmalloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) throw Pf5dError(PF5D_ENOMEM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+  return p;
+}
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+struct DevInfo {
+  int cus = 256, lds = 65536, warp = 64;
+};
+static DevInfo& devinfo() {
+  static DevInfo d;
+  static bool init = false;
+  if (!init) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    d.cus = p.multiProcessorCount;
+    d.lds = (int)p.sharedMemPerBlock;
+    d.warp = p.warpSize;
+    init = true;
+  }
+  return d;
+}
+
+static inline unsigned nblocks(long n, int bs) { return (unsigned)std::max<long>(1, (n + bs - 1) / bs); }
+
+// library-internal grow-only device scratch for reductions
+struct Scratch {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void* get(size_t b) {
+    if (b > bytes) {
+      dfree(p);
+      p = dmalloc(b);
+      bytes = b;
+    }
+    return p;
+  }
+};
+static Scratch& scratch() {
+  static thread_local Scratch s;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+struct pf5d_table {
+  int prec, num;
+  double resolution;
+  void* data;
+};
+
+template <typename T>
+static TableView<T> view(const pf5d_table* t) {
+  TableView<T> v;
+  v.data = reinterpret_cast<const T*>(t->data);
+  v.num = t->num;
+  v.inv_step = (T)(1.0 / t->resolution);
+  v.thresh = (T)(1.0 - 4.0 * t->resolution - 1e-9);
+  v.step = t->resolution;
+  return v;
+}
+
+template <typename T>
+static void upload_table(pf5d_table* t, const double* grid, int nprops) {
+  constexpr int NP = TablePad<T>::value;
+  const long nn = (long)t->num * t->num * t->num * t->num;
+  std::vector<T> h((size_t)nn * NP, T(0));
+  for (long i = 0; i < nn; ++i)
+    for (int p = 0; p < nprops; ++p) h[i * NP + p] = (T)grid[i * nprops + p];
+  t->data = dmalloc(h.size() * sizeof(T));
+  HIPCHK(hipMemcpy(t->data, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------
+// FFT plans and batched pencil passes
+// ---------------------------------------------------------------------------
+static std::vector<int> factorize(int n) {
+  std::vector<int> r;
+  int m = n;
+  while (m % 8 == 0 && m >= 8) {
+    r.push_back(8);
+    m /= 8;
+  }
+  while (m % 4 == 0) {
+    r.push_back(4);
+    m /= 4;
+  }
+  while (m % 2 == 0) {
+    r.push_back(2);
+    m /= 2;
+  }
+  for (int p : {3, 5, 7}) {
+    while (m % p == 0) {
+      r.push_back(p);
+      m /= p;
+    }
+  }
+  for (int p = 11; m > 1; p += 2) {
+    while (m % p == 0) {
+      r.push_back(p);
+      m /= p;
+    }
+  }
+  return r;
+}
+
+// exp(-2 pi i m / N) with integer reduction and long-double evaluation
+static void root_of_unity(long m, long N, double& re, double& im) {
+  m %= N;
+  const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)N;
+  re = (double)cosl(a);
+  im = (double)sinl(a);
+}
+
+template <typename T>
+struct FftPlans {
+  std::map<int, int> index;  // n -> plan id
+  std::vector<LinePlanDev> plans;
+  std::vector<cx<T>> tw, roots;
+  LinePlanDev* d_plans = nullptr;
+  cx<T>* d_tw = nullptr;
+  cx<T>* d_roots = nullptr;
+  bool dirty = false;
+
+  int get(int n) {
+    auto it = index.find(n);
+    if (it != index.end()) return it->second;
+    std::vector<int> rad = factorize(n);
+    require((int)rad.size() <= FFT_MAXPASS, "FFT length has too many factors: " + std::to_string(n));
+    LinePlanDev pl{};
+    pl.n = n;
+    pl.npass = (int)rad.size();
+    long Ns = 1;
+    for (int p = 0; p < pl.npass; ++p) {
+      const int R = rad[p];
+      pl.radix[p] = R;
+      pl.tw_off[p] = (int)tw.size();
+      for (long k = 0; k < Ns; ++k)
+        for (int r = 1; r < R; ++r) {
+          double re, im;
+          root_of_unity((long)r * k, Ns * R, re, im);
+          tw.push_back({(T)re, (T)im});
+        }
+      pl.root_off[p] = (int)roots.size();
+      if (!(R == 2 || R == 3 || R == 4 || R == 5 || R == 8))
+        for (int m = 0; m < R; ++m) {
+          double re, im;
+          root_of_unity(m, R, re, im);
+          roots.push_back({(T)re, (T)im});
+        }
+      Ns *= R;
+    }
+    int id = (int)plans.size();
+    plans.push_back(pl);
+    index[n] = id;
+    dirty = true;
+    return id;
+  }
+  void upload() {
+    if (!dirty) return;
+    dfree(d_plans);
+    dfree(d_tw);
+    dfree(d_roots);
+    d_plans = (LinePlanDev*)dmalloc(plans.size() * sizeof(LinePlanDev));
+    d_tw = (cx<T>*)dmalloc(std::max<size_t>(1, tw.size()) * sizeof(cx<T>));
+    d_roots = (cx<T>*)dmalloc(std::max<size_t>(1, roots.size()) * sizeof(cx<T>));
+    HIPCHK(hipMemcpy(d_plans, plans.data(), plans.size() * sizeof(LinePlanDev), hipMemcpyHostToDevice));
+    if (!tw.empty()) HIPCHK(hipMemcpy(d_tw, tw.data(), tw.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+    if (!roots.empty())
+      HIPCHK(hipMemcpy(d_roots, roots.data(), roots.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+    dirty = false;
+  }
+  ~FftPlans() {
+    dfree(d_plans);
+    dfree(d_tw);
+    dfree(d_roots);
+  }
+};
+
+constexpr int FFT_NT = 256;
+constexpr size_t FFT_LDS_BUDGET = 64 * 1024;
+
+// the five stages of psi = irfftn(Bhat rfftn(s)) and their work items
+template <typename T>
+struct FftBatch {
+  // plans are owned per batch: their device arrays are baked into captured
+  // graphs, so they may only be reallocated when the owner rebuilds its graphs
+  FftPlans<T> plans;
+  std::vector<FftItem> stage[5];  // 0: rows R2C / 1D line, 1: y fwd, 2: x conv, 3: y inv, 4: rows C2R
+  size_t lds[5] = {0, 0, 0, 0, 0};
+  FftItem* d_items = nullptr;
+  int offs[6] = {0, 0, 0, 0, 0, 0};
+
+  static int pick_g(int n, long nlines) {
+    long g = (long)(FFT_LDS_BUDGET / (2 * (size_t)n * sizeof(cx<T>)));
+    g = std::max<long>(1, std::min<long>(g, 64));
+    return (int)std::min<long>(g, nlines);
+  }
+  void push(int st, FftItem it, int n) {
+    stage[st].push_back(it);
+    lds[st] = std::max(lds[st], 2 * (size_t)it.G * n * sizeof(cx<T>));
+  }
+  // lines [0, nlines) with line l at base + l*ls; split into groups of G
+  void add_lines(int st, FftItem proto, int n, long nlines, long line_unit_in, long line_unit_out, long line_unit_b) {
+    const int G = pick_g(n, nlines);
+    for (long l0 = 0; l0 < nlines; l0 += G) {
+      FftItem it = proto;
+      it.G = (int)std::min<long>(G, nlines - l0);
+      it.in = (const char*)proto.in + l0 * line_unit_in;
+      it.out = (char*)proto.out + l0 * line_unit_out;
+      it.bhat = proto.bhat ? (const char*)proto.bhat + l0 * line_unit_b : nullptr;
+      push(st, it, n);
+    }
+  }
+  // s (real, ncell) -> psi (real) through spectrum C (complex, rows of nh) and bhat (real, C layout)
+  void add_sim(int sim, int dim, const long* shape, const T* s, cx<T>* C, T* psi,
+               const T* bhat) {
+    const int nl = (int)shape[dim - 1];
+    const long nh = nl / 2 + 1;
+    const long rows = [&] {
+      long r = 1;
+      for (int a = 0; a < dim - 1; ++a) r *= shape[a];
+      return r;
+    }();
+    FftItem b{};
+    b.sim = sim;
+    if (dim == 1) {
+      b.kind = FK_LINE;
+      b.plan = plans.get(nl);
+      b.G = 1;
+      b.in = s;
+      b.out = psi;
+      b.bhat = bhat;
+      b.ps_in = 1;
+      b.ls_in = nl;
+      b.ps_out = 1;
+      b.ls_out = nl;
+      push(0, b, nl);
+      return;
+    }
+    // rows: R2C
+    b.kind = FK_R2C;
+    b.plan = plans.get(nl);
+    b.in = s;
+    b.out = C;
+    b.ps_in = 1;
+    b.ls_in = nl;
+    b.ps_out = 1;
+    b.ls_out = nh;
+    add_lines(0, b, nl, rows, sizeof(T) * nl, sizeof(cx<T>) * nh, 0);
+    // rows: C2R
+    b.kind = FK_C2R;
+    b.in = C;
+    b.out = psi;
+    b.ps_in = 1;
+    b.ls_in = nh;
+    b.ps_out = 1;
+    b.ls_out = nl;
+    add_lines(4, b, nl, rows, sizeof(cx<T>) * nh, sizeof(T) * nl, 0);
+    if (dim == 2) {
+      const int nx = (int)shape[0];
+      b.kind = FK_CONV;
+      b.plan = plans.get(nx);
+      b.in = C;
+      b.out = C;
+      b.bhat = bhat;
+      b.ps_in = b.ps_out = b.ps_b = nh;
+      b.ls_in = b.ls_out = b.ls_b = 1;
+      add_lines(2, b, nx, nh, sizeof(cx<T>), sizeof(cx<T>), sizeof(T));
+      return;
+    }
+    const int nx = (int)shape[0], ny = (int)shape[1];
+    // y passes: for each x plane, lines kz in [0, nh) with point stride nh
+    for (int x = 0; x < nx; ++x) {
+      FftItem y{};
+      y.sim = sim;
+      y.plan = plans.get(ny);
+      y.in = C + (long)x * ny * nh;
+      y.out = C + (long)x * ny * nh;
+      y.ps_in = y.ps_out = nh;
+      y.ls_in = y.ls_out = 1;
+      y.kind = FK_FWD;
+      add_lines(1, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0);
+      y.kind = FK_INV;
+      add_lines(3, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0);
+    }
+    // x pass with Bhat: lines (y, kz) flattened, point stride ny*nh
+    b.kind = FK_CONV;
+    b.plan = plans.get(nx);
+    b.in = C;
+    b.out = C;
+    b.bhat = bhat;
+    b.ps_in = b.ps_out = b.ps_b = (long)ny * nh;
+    b.ls_in = b.ls_out = b.ls_b = 1;
+    add_lines(2, b, nx, (long)ny * nh, sizeof(cx<T>), sizeof(cx<T>), sizeof(T));
+  }
+  void clear() {
+    for (auto& s : stage) s.clear();
+    for (auto& l : lds) l = 0;
+  }
+  void upload() {
+    dfree(d_items);
+    d_items = nullptr;
+    std::vector<FftItem> all;
+    for (int i = 0; i < 5; ++i) {
+      offs[i] = (int)all.size();
+      all.insert(all.end(), stage[i].begin(), stage[i].end());
+    }
+    offs[5] = (int)all.size();
+    if (all.empty()) return;
+    d_items = (FftItem*)dmalloc(all.size() * sizeof(FftItem));
+    HIPCHK(hipMemcpy(d_items, all.data(), all.size() * sizeof(FftItem), hipMemcpyHostToDevice));
+  }
+  void run(hipStream_t stream, const SimState* st) {
+    const DevInfo& di = devinfo();
+    for (int i = 0; i < 5; ++i) {
+      const int n = offs[i + 1] - offs[i];
+      if (n == 0) continue;
+      require(lds[i] <= (size_t)di.lds, "FFT line too long for LDS: " + std::to_string(lds[i]) + " bytes");
+      const unsigned grid = (unsigned)std::min<long>(n, (long)di.cus * 8);
+      hipLaunchKernelGGL((k_fft<T, FFT_NT>), dim3(grid), dim3(FFT_NT), lds[i], stream, d_items + offs[i], n,
+                         plans.d_plans, plans.d_tw, plans.d_roots, st);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  ~FftBatch() { dfree(d_items); }
+};
+
+// Bhat (k_shape, double) -> device T in the spectrum layout, with 1/N folded in
+template <typename T>
+static T* upload_bhat(int dim, const long* shape, const double* bhat_host) {
+  long nk = 1, ncell = 1;
+  for (int a = 0; a < dim; ++a) ncell *= shape[a];
+  for (int a = 0; a < dim - 1; ++a) nk *= shape[a];
+  nk *= shape[dim - 1] / 2 + 1;
+  std::vector<T> h(nk);
+  const double inv = 1.0 / (double)ncell;
+  for (long i = 0; i < nk; ++i) h[i] = (T)(bhat_host[i] * inv);
+  T* d = (T*)dmalloc(nk * sizeof(T));
+  HIPCHK(hipMemcpy(d, h.data(), nk * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+static long spectrum_size(int dim, const long* shape) {
+  long nk = 1;
+  for (int a = 0; a < dim - 1; ++a) nk *= shape[a];
+  return nk * (shape[dim - 1] / 2 + 1);
+}
+
+// ---------------------------------------------------------------------------
+// plain elastic updater
+// ---------------------------------------------------------------------------
+struct pf5d_elastic {
+  int prec, dim;
+  long shape[3], ncell;
+  double l[4], X0[4];
+  void *s = nullptr, *C = nullptr, *psi = nullptr, *bhat = nullptr;
+  std::unique_ptr<FftBatch<double>> b64;
+  std::unique_ptr<FftBatch<float>> b32;
+  ~pf5d_elastic() {
+    dfree(s);
+    dfree(C);
+    dfree(psi);
+    dfree(bhat);
+  }
+};
+
+template <typename T>
+static void elastic_init(pf5d_elastic* e, const double* bhat_host) {
+  e->s = dmalloc(e->ncell * sizeof(T));
+  e->psi = dmalloc(e->ncell * sizeof(T));
+  e->C = dmalloc(spectrum_size(e->dim, e->shape) * sizeof(cx<T>));
+  e->bhat = upload_bhat<T>(e->dim, e->shape, bhat_host);
+  auto b = std::make_unique<FftBatch<T>>();
+  b->add_sim(0, e->dim, e->shape, (const T*)e->s, (cx<T>*)e->C, (T*)e->psi, (const T*)e->bhat);
+  b->plans.upload();
+  b->upload();
+  if constexpr (sizeof(T) == 8)
+    e->b64 = std::move(b);
+  else
+    e->b32 = std::move(b);
+}
+
+template <typename T>
+static void elastic_apply(pf5d_elastic* e, void* mu, long mstride, const void* X, hipStream_t st) {
+  V4<T> x0, l;
+  for (int a = 0; a < 4; ++a) {
+    x0.v[a] = (T)e->X0[a];
+    l.v[a] = (T)e->l[a];
+  }
+  hipLaunchKernelGGL(k_misfit<T>, dim3(nblocks(e->ncell, 256)), dim3(256), 0, st, (const T*)X, e->ncell, x0, l,
+                     (T*)e->s);
+  HIPCHK(hipGetLastError());
+  if constexpr (sizeof(T) == 8)
+    e->b64->run(st, nullptr);
+  else
+    e->b32->run(st, nullptr);
+  hipLaunchKernelGGL(k_add_psi<T>, dim3(nblocks(e->ncell, 256)), dim3(256), 0, st, (T*)mu, mstride,
+                     (const T*)e->psi, e->ncell, l);
+  HIPCHK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+struct EngineBase {
+  virtual ~EngineBase() {}
+  virtual void add(const pf5d_sim_desc* d, const double* X, const double* bhat, const pf5d_table* tab, int* slot) = 0;
+  virtual void remove(int slot) = 0;
+  virtual void step(int n) = 0;
+  virtual void get_state(int slot, double* out) = 0;
+  virtual void set_time(int slot, double dt, double t) = 0;
+  virtual void get_field(int slot, int which, double* out) = 0;
+  virtual void set_field(int slot, const double* X) = 0;
+  virtual void* field_ptr(int slot, int which) = 0;
+  virtual void stats(long* active, long* bytes) = 0;
+  virtual void set_tiling(int variant, int cx) = 0;
+  virtual int num_variants(int* info) = 0;
+  virtual void launch_info(long* out) = 0;
+  virtual double time_kernel(int which, int reps) = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+};
+
+struct pf5d_engine {
+  int prec;
+  std::unique_ptr<EngineBase> impl;
+};
+
+template <typename T>
+struct K1Variant {
+  int by, bz, nt;
+  size_t lds;
+  void (*fn)(const SimDev<T>*, SimState*, T*, const K1Item*, int);
+};
+
+template <typename T, int BY, int BZ, int NT>
+static K1Variant<T> make_variant() {
+  return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT>};
+}
+
+template <typename T>
+static std::vector<K1Variant<T>> k1_variants();
+template <>
+std::vector<K1Variant<double>> k1_variants<double>() {
+  return {make_variant<double, 8, 32, 256>(), make_variant<double, 4, 32, 128>(), make_variant<double, 8, 16, 128>()};
+}
+template <>
+std::vector<K1Variant<float>> k1_variants<float>() {
+  return {make_variant<float, 8, 32, 256>(), make_variant<float, 16, 32, 256>(), make_variant<float, 8, 64, 256>(),
+          make_variant<float, 4, 64, 256>()};
+}
+
+constexpr int K2_NT = 256;
+constexpr long K2_CHUNK = 8192;
+constexpr int DT_NT = 256;
+
+template <typename T>
+struct Engine : EngineBase {
+  struct Sim {
+    bool used = false;
+    pf5d_sim_desc d{};
+    long ncell = 0;
+    T *X = nullptr, *dX = nullptr, *psi = nullptr, *s = nullptr, *bhat = nullptr;
+    cx<T>* C = nullptr;
+    const pf5d_table* tab = nullptr;
+    size_t bytes = 0;
+    bool fresh = true;
+  };
+  std::vector<Sim> sims;
+  std::vector<K1Variant<T>> variants = k1_variants<T>();
+  int variant = 0;
+  int xchunk = 16;
+  // device arrays
+  SimDev<T>* d_sims = nullptr;
+  SimState* d_state = nullptr;
+  T* d_xmax = nullptr;
+  K1Item* d_k1 = nullptr;
+  K2Item* d_k2 = nullptr;
+  double* d_part = nullptr;
+  int* d_prange = nullptr;
+  int nk1 = 0, nk2 = 0, cap = 0;
+  FftBatch<T> fft;
+  bool dirty = true;
+  hipGraphExec_t g1 = nullptr, gN = nullptr;
+  static constexpr int GN = 8;
+  unsigned k1_grid = 0;
+
+  Engine(hipStream_t s) {
+    if (s) {
+      stream = s;
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      own_stream = true;
+    }
+    for (auto& v : variants) (void)hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds);
+    variant = -1;
+    for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
+      if (variants[i].lds <= (size_t)devinfo().lds) variant = i;
+    require(variant >= 0, "no stencil tiling fits this device's LDS");
+    (void)hipFuncSetAttribute((const void*)&k_fft<T, FFT_NT>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+  }
+  ~Engine() override {
+    drop_graphs();
+    for (auto& s : sims) free_sim(s);
+    dfree(d_sims);
+    dfree(d_state);
+    dfree(d_xmax);
+    dfree(d_k1);
+    dfree(d_k2);
+    dfree(d_part);
+    dfree(d_prange);
+    if (own_stream) (void)hipStreamDestroy(stream);
+  }
+  void free_sim(Sim& s) {
+    dfree(s.X);
+    dfree(s.dX);
+    dfree(s.psi);
+    dfree(s.s);
+    dfree(s.bhat);
+    dfree(s.C);
+    s = Sim();
+  }
+  void drop_graphs() {
+    if (g1) (void)hipGraphExecDestroy(g1);
+    if (gN) (void)hipGraphExecDestroy(gN);
+    g1 = gN = nullptr;
+  }
+  void ensure_capacity(int n) {
+    if (n <= cap) return;
+    int nc = std::max(n, std::max(8, 2 * cap));
+    SimState* ns = (SimState*)dmalloc(nc * sizeof(SimState));
+    HIPCHK(hipMemset(ns, 0, nc * sizeof(SimState)));
+    if (d_state) HIPCHK(hipMemcpy(ns, d_state, cap * sizeof(SimState), hipMemcpyDeviceToDevice));
+    dfree(d_state);
+    d_state = ns;
+    T* nx = (T*)dmalloc(nc * sizeof(T));
+    HIPCHK(hipMemset(nx, 0, nc * sizeof(T)));
+    dfree(d_xmax);
+    d_xmax = nx;
+    dfree(d_sims);
+    d_sims = (SimDev<T>*)dmalloc(nc * sizeof(SimDev<T>));
+    dfree(d_prange);
+    d_prange = (int*)dmalloc(nc * 2 * sizeof(int));
+    cap = nc;
+  }
+  void add(const pf5d_sim_desc* d, const double* Xh, const double* bhat, const pf5d_table* tab, int* slot) override {
+    require(d->dim >= 1 && d->dim <= 3, "dim must be 1, 2 or 3", PF5D_ENOTIMPL);
+    require(tab && tab->prec == (sizeof(T) == 8 ? PF5D_F64 : PF5D_F32), "table precision does not match engine");
+    long ncell = 1;
+    for (int a = 0; a < d->dim; ++a) {
+      require(d->shape[a] >= 4, "every axis needs at least 4 cells");
+      ncell *= d->shape[a];
+    }
+    require(!d->elastic || bhat, "elastic simulation needs bhat");
+    HIPCHK(hipStreamSynchronize(stream));
+    int id = -1;
+    for (int i = 0; i < (int)sims.size(); ++i)
+      if (!sims[i].used) {
+        id = i;
+        break;
+      }
+    if (id < 0) {
+      id = (int)sims.size();
+      sims.emplace_back();
+    }
+    ensure_capacity((int)sims.size());
+    Sim& s = sims[id];
+    s.used = true;
+    s.d = *d;
+    s.ncell = ncell;
+    s.tab = tab;
+    s.fresh = true;
+    try {
+      s.X = (T*)dmalloc(ncell * 4 * sizeof(T));
+      s.dX = (T*)dmalloc(ncell * 4 * sizeof(T));
+      s.bytes = ncell * 8 * sizeof(T);
+      if (d->elastic) {
+        s.psi = (T*)dmalloc(ncell * sizeof(T));
+        s.s = (T*)dmalloc(ncell * sizeof(T));
+        s.C = (cx<T>*)dmalloc(spectrum_size(d->dim, d->shape) * sizeof(cx<T>));
+        s.bhat = upload_bhat<T>(d->dim, d->shape, bhat);
+        s.bytes += ncell * 2 * sizeof(T) + spectrum_size(d->dim, d->shape) * (sizeof(cx<T>) + sizeof(T));
+      }
+      upload_X(s, Xh);
+      HIPCHK(hipMemset(s.dX, 0, ncell * 4 * sizeof(T)));
+    } catch (...) {
+      free_sim(s);
+      throw;
+    }
+    SimState st{};
+    st.dt = d->dt0;
+    st.t = 0.0;
+    st.status = ST_ACTIVE;
+    HIPCHK(hipMemcpy(d_state + id, &st, sizeof(SimState), hipMemcpyHostToDevice));
+    *slot = id;
+    dirty = true;
+  }
+  void upload_X(Sim& s, const double* Xh) {
+    std::vector<T> h(s.ncell * 4);
+    for (long i = 0; i < s.ncell * 4; ++i) h[i] = (T)Xh[i];
+    HIPCHK(hipMemcpy(s.X, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  Sim& at(int slot) {
+    require(slot >= 0 && slot < (int)sims.size() && sims[slot].used, "invalid slot " + std::to_string(slot));
+    return sims[slot];
+  }
+  void remove(int slot) override {
+    Sim& s = at(slot);
+    HIPCHK(hipStreamSynchronize(stream));
+    free_sim(s);
+    SimState st{};
+    HIPCHK(hipMemcpy(d_state + slot, &st, sizeof(SimState), hipMemcpyHostToDevice));
+    dirty = true;
+  }
+  SimDev<T> make_dev(const Sim& s) {
+    SimDev<T> v{};
+    const pf5d_sim_desc& d = s.d;
+    v.dim = d.dim;
+    if (d.dim == 3) {
+      v.n0 = (int)d.shape[0];
+      v.n1 = (int)d.shape[1];
+      v.n2 = (int)d.shape[2];
+    } else if (d.dim == 2) {
+      v.n0 = 1;
+      v.n1 = (int)d.shape[0];
+      v.n2 = (int)d.shape[1];
+    } else {
+      v.n0 = 1;
+      v.n1 = 1;
+      v.n2 = (int)d.shape[0];
+    }
+    v.ncell = s.ncell;
+    v.inv_dx = (T)(1.0 / d.cell_size);
+    v.inv_dx2 = (T)(1.0 / (d.cell_size * d.cell_size));
+    for (int a = 0; a < 4; ++a) {
+      v.kappa[a] = (T)d.kappa[a];
+      v.l[a] = d.elastic ? (T)d.l[a] : T(0);
+      v.X0[a] = (T)d.X0[a];
+    }
+    v.elastic = d.elastic;
+    v.use_mean = (sizeof(T) == 4) ? 1 : 0;
+    v.tab = view<T>(s.tab);
+    v.X = s.X;
+    v.dX = s.dX;
+    v.psi = s.psi;
+    v.s = s.s;
+    v.ctld = d.ctld;
+    v.dt_cfl = d.dt_cfl;
+    v.lower = d.lower_limit;
+    v.upper = d.upper_limit;
+    v.t_end = d.t_end;
+    v.oob_interval = d.oob_interval;
+    v.tol_soft = d.tol_soft;
+    v.tol_hard = d.tol_hard;
+    return v;
+  }
+  void rebuild() {
+    HIPCHK(hipStreamSynchronize(stream));  // buffers below may still be in use
+    drop_graphs();
+    const K1Variant<T>& V = variants[variant];
+    std::vector<SimDev<T>> devs(cap);
+    std::vector<K1Item> k1;
+    std::vector<K2Item> k2;
+    std::vector<int> prange(2 * cap, 0);
+    fft.clear();
+    for (int i = 0; i < (int)sims.size(); ++i) {
+      Sim& s = sims[i];
+      if (!s.used) continue;
+      devs[i] = make_dev(s);
+      const SimDev<T>& v = devs[i];
+      const bool hx = v.dim == 3;
+      const int by = (v.dim >= 2) ? V.by : 1;
+      const int cxs = hx ? std::max(1, std::min(xchunk, v.n0)) : 1;
+      for (int xa = 0; xa < (hx ? v.n0 : 1); xa += cxs)
+        for (int y0 = 0; y0 < v.n1; y0 += by)
+          for (int z0 = 0; z0 < v.n2; z0 += V.bz) {
+            K1Item it;
+            it.sim = i;
+            it.xa = xa;
+            it.xb = hx ? std::min(xa + cxs, v.n0) : 1;
+            it.y0 = y0;
+            it.z0 = z0;
+            it.ty = std::min(by, v.n1 - y0);
+            it.tz = std::min(V.bz, v.n2 - z0);
+            k1.push_back(it);
+          }
+      prange[2 * i] = (int)k2.size();
+      for (long c0 = 0; c0 < s.ncell; c0 += K2_CHUNK) k2.push_back({i, c0, std::min(s.ncell, c0 + K2_CHUNK)});
+      prange[2 * i + 1] = (int)k2.size();
+      if (s.d.elastic)
+        fft.add_sim(i, s.d.dim, s.d.shape, s.s, s.C, s.psi, s.bhat);
+    }
+    fft.plans.upload();
+    fft.upload();
+    HIPCHK(hipMemcpy(d_sims, devs.data(), cap * sizeof(SimDev<T>), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_prange, prange.data(), prange.size() * sizeof(int), hipMemcpyHostToDevice));
+    dfree(d_k1);
+    dfree(d_k2);
+    dfree(d_part);
+    d_k1 = nullptr;
+    d_k2 = nullptr;
+    d_part = nullptr;
+    nk1 = (int)k1.size();
+    nk2 = (int)k2.size();
+    if (nk1) {
+      d_k1 = (K1Item*)dmalloc(nk1 * sizeof(K1Item));
+      HIPCHK(hipMemcpy(d_k1, k1.data(), nk1 * sizeof(K1Item), hipMemcpyHostToDevice));
+    }
+    if (nk2) {
+      d_k2 = (K2Item*)dmalloc(nk2 * sizeof(K2Item));
+      HIPCHK(hipMemcpy(d_k2, k2.data(), nk2 * sizeof(K2Item), hipMemcpyHostToDevice));
+      d_part = (double*)dmalloc((size_t)nk2 * NSTAT * sizeof(double));
+    }
+    // K1 grid: persistent, at most what fits on the chip at once
+    int occ = 1;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
+    occ = std::max(1, occ);
+    k1_grid = (unsigned)std::max(1, std::min(nk1, devinfo().cus * occ));
+    // prime new simulations: misfit + statistics, mean, misfit about the mean
+    bool any_fresh = false;
+    for (auto& s : sims) any_fresh = any_fresh || (s.used && s.fresh);
+    if (any_fresh && nk2) {
+      launch_k2(1);
+      launch_dt(0);
+      launch_k2(1);
+      launch_dt(0);
+      for (auto& s : sims) s.fresh = false;
+    }
+    dirty = false;
+  }
+  void launch_k2(int prime) {
+    if (!nk2) return;
+    const unsigned grid = (unsigned)std::min(nk2, devinfo().cus * 8);
+    hipLaunchKernelGGL((k_update<T, K2_NT>), dim3(grid), dim3(K2_NT), 0, stream, d_sims, d_state, d_k2, nk2, d_part,
+                       prime);
+    HIPCHK(hipGetLastError());
+  }
+  void launch_dt(int advance) {
+    hipLaunchKernelGGL(k_dtctl<T>, dim3((unsigned)sims.size()), dim3(DT_NT), 0, stream, d_sims, d_state, d_xmax,
+                       d_part, d_prange, (int)sims.size(), advance);
+    HIPCHK(hipGetLastError());
+  }
+  void launch_k1() {
+    if (!nk1) return;
+    const K1Variant<T>& V = variants[variant];
+    require(V.lds <= (size_t)devinfo().lds, "K1 tile needs more LDS than a block may use");
+    hipLaunchKernelGGL(V.fn, dim3(k1_grid), dim3(V.nt), V.lds, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+    HIPCHK(hipGetLastError());
+  }
+  void enqueue_step() {
+    fft.run(stream, d_state);
+    launch_k1();
+    launch_dt(1);
+    launch_k2(0);
+  }
+  hipGraphExec_t capture(int n) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int i = 0; i < n; ++i) enqueue_step();
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream, &g));
+    hipGraphExec_t ex;
+    HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    return ex;
+  }
+  void step(int n) override {
+    require(n >= 0, "nsteps must be >= 0");
+    if (dirty) rebuild();
+    if (nk1 == 0 || n == 0) return;
+    if (!g1) g1 = capture(1);
+    if (n >= GN && !gN) gN = capture(GN);
+    while (n >= GN) {
+      HIPCHK(hipGraphLaunch(gN, stream));
+      n -= GN;
+    }
+    while (n-- > 0) HIPCHK(hipGraphLaunch(g1, stream));
+  }
+  void get_state(int slot, double* out) override {
+    at(slot);
+    if (dirty) rebuild();
+    SimState st;
+    HIPCHK(hipMemcpyAsync(&st, d_state + slot, sizeof(SimState), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    const double v[16] = {st.dt,  st.t,    st.last_xmax, st.mass[0], st.mass[1], st.mass[2],     st.mass[3], (double)st.steps,
+                          (double)st.status, st.smax, st.smin, st.cmin, st.cmax, st.nonfinite, st.mean[0], st.mean[1]};
+    std::memcpy(out, v, sizeof(v));
+  }
+  void set_time(int slot, double dt, double t) override {
+    at(slot);
+    if (dirty) rebuild();
+    HIPCHK(hipStreamSynchronize(stream));
+    SimState st;
+    HIPCHK(hipMemcpy(&st, d_state + slot, sizeof(SimState), hipMemcpyDeviceToHost));
+    st.dt = dt;
+    st.t = t;
+    HIPCHK(hipMemcpy(d_state + slot, &st, sizeof(SimState), hipMemcpyHostToDevice));
+  }
+  void get_field(int slot, int which, double* out) override {
+    Sim& s = at(slot);
+    const T* src = which == 0 ? s.X : which == 1 ? s.dX : s.psi;
+    require(src != nullptr, "field not present");
+    const long n = (which == 2) ? s.ncell : s.ncell * 4;
+    std::vector<T> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (long i = 0; i < n; ++i) out[i] = (double)h[i];
+  }
+  void set_field(int slot, const double* X) override {
+    Sim& s = at(slot);
+    HIPCHK(hipStreamSynchronize(stream));
+    upload_X(s, X);
+    s.fresh = true;
+    dirty = true;
+  }
+  void* field_ptr(int slot, int which) override {
+    Sim& s = at(slot);
+    return which == 0 ? (void*)s.X : which == 1 ? (void*)s.dX : (void*)s.psi;
+  }
+  void stats(long* active, long* bytes) override {
+    long a = 0, b = 0;
+    for (auto& s : sims)
+      if (s.used) {
+        ++a;
+        b += (long)s.bytes;
+      }
+    *active = a;
+    *bytes = b;
+  }
+  void set_tiling(int v, int c) override {
+    require(v >= 0 && v < (int)variants.size(), "tiling variant out of range");
+    require(c >= 1, "cx must be >= 1");
+    variant = v;
+    xchunk = c;
+    dirty = true;
+  }
+  int num_variants(int* info) override {
+    if (info)
+      for (size_t i = 0; i < variants.size(); ++i) {
+        info[3 * i] = variants[i].by;
+        info[3 * i + 1] = variants[i].bz;
+        info[3 * i + 2] = variants[i].nt;
+      }
+    return (int)variants.size();
+  }
+  // average duration of one launch of a single stage, timed with HIP events on
+  // the engine stream: 0 = K1 stencil, 1 = FFT stages (all five), 2 = DT, 3 = K2 prime
+  double time_kernel(int which, int reps) override {
+    require(reps > 0 && which >= 0 && which <= 3, "time_kernel: bad arguments");
+    if (dirty) rebuild();
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    auto one = [&] {
+      if (which == 0) launch_k1();
+      else if (which == 1) fft.run(stream, d_state);
+      else if (which == 2) launch_dt(0);
+      else launch_k2(1);
+    };
+    one();
+    HIPCHK(hipEventRecord(a, stream));
+    for (int i = 0; i < reps; ++i) one();
+    HIPCHK(hipEventRecord(b, stream));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return (double)ms / reps;
+  }
+  void launch_info(long* out) override {
+    out[0] = nk1;
+    out[1] = nk2;
+    out[2] = fft.offs[5];
+    out[3] = k1_grid;
+    out[4] = (long)variants[variant].lds;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+static hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+static void check_prec(int p) { require(p == PF5D_F64 || p == PF5D_F32, "prec must be PF5D_F64 or PF5D_F32"); }
+static Shape make_shape(int dim, const long* shape) {
+  if (dim < 1 || dim > 3) throw Pf5dError(PF5D_ENOTIMPL, "dimension must be 1, 2 or 3");
+  Shape s;
+  s.dim = dim;
+  s.ncell = 1;
+  for (int a = 0; a < 3; ++a) s.n[a] = a < dim ? (int)shape[a] : 1;
+  for (int a = 0; a < dim; ++a) {
+    require(shape[a] >= 1, "shape entries must be positive");
+    s.ncell *= shape[a];
+  }
+  return s;
+}
+
+extern "C" {
+
+const char* pf5d_last_error(void) { return g_err.c_str(); }
+int pf5d_version(void) { return 1; }
+
+int pf5d_device_info(int out[3]) {
+  return guarded([&] {
+    const DevInfo& d = devinfo();
+    out[0] = d.cus;
+    out[1] = d.lds;
+    out[2] = d.warp;
+  });
+}
+
+int pf5d_table_create(int prec, int num, int nprops, const double* grid, double resolution, pf5d_table** out) {
+  return guarded([&] {
+    check_prec(prec);
+    require(num >= 2 && nprops == 9 && grid && out && resolution > 0, "table: need num >= 2, nprops == 9");
+    auto t = std::make_unique<pf5d_table>();
+    t->prec = prec;
+    t->num = num;
+    t->resolution = resolution;
+    t->data = nullptr;
+    if (prec == PF5D_F64)
+      upload_table<double>(t.get(), grid, nprops);
+    else
+      upload_table<float>(t.get(), grid, nprops);
+    *out = t.release();
+  });
+}
+int pf5d_table_destroy(pf5d_table* t) {
+  return guarded([&] {
+    if (!t) return;
+    dfree(t->data);
+    delete t;
+  });
+}
+
+int pf5d_interpolate(int prec, const void* X, long ncell, const pf5d_table* tab, void* out, long ostride,
+                     void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    require(tab && tab->prec == prec, "table precision mismatch");
+    require(ostride >= 9, "output row stride must be >= 9");
+    if (ncell <= 0) return;
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_interp<double>, dim3(nblocks(ncell, 256)), dim3(256), 0, S(stream), (const double*)X,
+                         ncell, view<double>(tab), (double*)out, ostride);
+    else
+      hipLaunchKernelGGL(k_interp<float>, dim3(nblocks(ncell, 256)), dim3(256), 0, S(stream), (const float*)X, ncell,
+                         view<float>(tab), (float*)out, ostride);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_laplacian(int prec, const void* X, int dim, const long* shape, double h, void* out, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    Shape sh = make_shape(dim, shape);
+    const double inv = 1.0 / (h * h);
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_laplacian<double>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream),
+                         (const double*)X, sh, inv, (double*)out);
+    else
+      hipLaunchKernelGGL(k_laplacian<float>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream), (const float*)X,
+                         sh, (float)inv, (float*)out);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_kappa_subtract(int prec, void* L, void* mu, long mstride, long ncell, const double kappa[4], void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    if (ncell <= 0) return;
+    if (prec == PF5D_F64) {
+      V4<double> k{{kappa[0], kappa[1], kappa[2], kappa[3]}};
+      hipLaunchKernelGGL(k_kappa_sub<double>, dim3(nblocks(ncell, 256)), dim3(256), 0, S(stream), (double*)L,
+                         (double*)mu, mstride, ncell, k);
+    } else {
+      V4<float> k{{(float)kappa[0], (float)kappa[1], (float)kappa[2], (float)kappa[3]}};
+      hipLaunchKernelGGL(k_kappa_sub<float>, dim3(nblocks(ncell, 256)), dim3(256), 0, S(stream), (float*)L,
+                         (float*)mu, mstride, ncell, k);
+    }
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_divergence_from_mu(int prec, const void* X, const void* mob, long bstride, const void* mu, long mstride,
+                            int dim, const long* shape, double inv_dx, void* out, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    Shape sh = make_shape(dim, shape);
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_div_from_mu<double>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream),
+                         (const double*)X, (const double*)mob, bstride, (const double*)mu, mstride, sh, inv_dx,
+                         (double*)out);
+    else
+      hipLaunchKernelGGL(k_div_from_mu<float>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream),
+                         (const float*)X, (const float*)mob, bstride, (const float*)mu, mstride, sh, (float)inv_dx,
+                         (float*)out);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_abs_max(int prec, const void* a, long n, double* host_result, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    void* d = scratch().get(16);
+    const unsigned grid = (unsigned)std::min<long>(nblocks(n, 256), 4L * devinfo().cus);
+    double r = 0;
+    if (prec == PF5D_F64) {
+      hipLaunchKernelGGL(k_fill<double>, dim3(1), dim3(64), 0, S(stream), (double*)d, 1L, 0.0);
+      if (n > 0)
+        hipLaunchKernelGGL(k_absmax<double>, dim3(grid), dim3(256), 0, S(stream), (const double*)a, n, (double*)d);
+      HIPCHK(hipMemcpyAsync(&r, d, sizeof(double), hipMemcpyDeviceToHost, S(stream)));
+      HIPCHK(hipStreamSynchronize(S(stream)));
+    } else {
+      float f = 0;
+      hipLaunchKernelGGL(k_fill<float>, dim3(1), dim3(64), 0, S(stream), (float*)d, 1L, 0.0f);
+      if (n > 0)
+        hipLaunchKernelGGL(k_absmax<float>, dim3(grid), dim3(256), 0, S(stream), (const float*)a, n, (float*)d);
+      HIPCHK(hipMemcpyAsync(&f, d, sizeof(float), hipMemcpyDeviceToHost, S(stream)));
+      HIPCHK(hipStreamSynchronize(S(stream)));
+      r = f;
+    }
+    HIPCHK(hipGetLastError());
+    *host_result = r;
+  });
+}
+
+int pf5d_update(int prec, void* X, void* dX, long n, double dt, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    if (n <= 0) return;
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_update_plain<double>, dim3(nblocks(n, 256)), dim3(256), 0, S(stream), (double*)X,
+                         (double*)dX, n, dt);
+    else
+      hipLaunchKernelGGL(k_update_plain<float>, dim3(nblocks(n, 256)), dim3(256), 0, S(stream), (float*)X,
+                         (float*)dX, n, dt);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_clamp_simplex(int prec, void* X, long ncell, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    if (ncell <= 0) return;
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_clamp<double>, dim3(nblocks(ncell, 256)), dim3(256), 0, S(stream), (double*)X, ncell);
+    else
+      hipLaunchKernelGGL(k_clamp<float>, dim3(nblocks(ncell, 256)), dim3(256), 0, S(stream), (float*)X, ncell);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_bounds_stats(int prec, const void* X, long ncell, double out5[5], void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    const unsigned grid = (unsigned)std::min<long>(nblocks(ncell, 256), 2L * devinfo().cus);
+    double* d = (double*)scratch().get(grid * 5 * sizeof(double));
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_oob_partial<double>, dim3(grid), dim3(256), 0, S(stream), (const double*)X, ncell, d);
+    else
+      hipLaunchKernelGGL(k_oob_partial<float>, dim3(grid), dim3(256), 0, S(stream), (const float*)X, ncell, d);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(grid * 5);
+    HIPCHK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, S(stream)));
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    double v[5] = {0, -1e300, 1e300, 1e300, -1e300};
+    for (unsigned b = 0; b < grid; ++b) {
+      v[0] += h[b * 5];
+      v[1] = std::max(v[1], h[b * 5 + 1]);
+      v[2] = std::min(v[2], h[b * 5 + 2]);
+      v[3] = std::min(v[3], h[b * 5 + 3]);
+      v[4] = std::max(v[4], h[b * 5 + 4]);
+    }
+    std::memcpy(out5, v, sizeof(v));
+  });
+}
+
+int pf5d_elastic_create(int prec, int dim, const long* shape, const double* bhat, const double l[4],
+                        const double X0[4], pf5d_elastic** out) {
+  return guarded([&] {
+    check_prec(prec);
+    Shape sh = make_shape(dim, shape);
+    require(bhat && l && X0 && out, "elastic: null argument");
+    auto e = std::make_unique<pf5d_elastic>();
+    e->prec = prec;
+    e->dim = dim;
+    for (int a = 0; a < 3; ++a) e->shape[a] = a < dim ? shape[a] : 1;
+    e->ncell = sh.ncell;
+    for (int a = 0; a < 4; ++a) {
+      e->l[a] = l[a];
+      e->X0[a] = X0[a];
+    }
+    if (prec == PF5D_F64)
+      elastic_init<double>(e.get(), bhat);
+    else
+      elastic_init<float>(e.get(), bhat);
+    *out = e.release();
+  });
+}
+int pf5d_elastic_apply(pf5d_elastic* e, void* mu, long mstride, const void* X, void* stream) {
+  return guarded([&] {
+    require(e != nullptr, "null elastic handle");
+    if (e->prec == PF5D_F64)
+      elastic_apply<double>(e, mu, mstride, X, S(stream));
+    else
+      elastic_apply<float>(e, mu, mstride, X, S(stream));
+  });
+}
+const void* pf5d_elastic_psi(const pf5d_elastic* e) { return e ? e->psi : nullptr; }
+int pf5d_elastic_destroy(pf5d_elastic* e) {
+  return guarded([&] { delete e; });
+}
+
+int pf5d_engine_create(int prec, void* stream, pf5d_engine** out) {
+  return guarded([&] {
+    check_prec(prec);
+    require(out != nullptr, "null output");
+    auto e = std::make_unique<pf5d_engine>();
+    e->prec = prec;
+    if (prec == PF5D_F64)
+      e->impl = std::make_unique<Engine<double>>(S(stream));
+    else
+      e->impl = std::make_unique<Engine<float>>(S(stream));
+    *out = e.release();
+  });
+}
+int pf5d_engine_destroy(pf5d_engine* e) {
+  return guarded([&] {
+    if (e && e->impl) (void)hipStreamSynchronize(e->impl->stream);
+    delete e;
+  });
+}
+int pf5d_engine_add(pf5d_engine* e, const pf5d_sim_desc* d, const double* X, const double* bhat,
+                    const pf5d_table* tab, int* slot) {
+  return guarded([&] {
+    require(e && d && X && slot, "engine_add: null argument");
+    e->impl->add(d, X, bhat, tab, slot);
+  });
+}
+int pf5d_engine_remove(pf5d_engine* e, int slot) {
+  return guarded([&] { e->impl->remove(slot); });
+}
+int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* info) {
+  return guarded([&] { *n = e->impl->num_variants(info); });
+}
+int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx) {
+  return guarded([&] { e->impl->set_tiling(variant, cx); });
+}
+int pf5d_engine_step(pf5d_engine* e, int nsteps) {
+  return guarded([&] { e->impl->step(nsteps); });
+}
+int pf5d_engine_sync(pf5d_engine* e) {
+  return guarded([&] { HIPCHK(hipStreamSynchronize(e->impl->stream)); });
+}
+int pf5d_engine_get_state(pf5d_engine* e, int slot, double* st) {
+  return guarded([&] { e->impl->get_state(slot, st); });
+}
+int pf5d_engine_set_time(pf5d_engine* e, int slot, double dt, double t) {
+  return guarded([&] { e->impl->set_time(slot, dt, t); });
+}
+int pf5d_engine_get_field(pf5d_engine* e, int slot, int which, double* out) {
+  return guarded([&] {
+    require(which >= 0 && which <= 2, "which must be 0, 1 or 2");
+    e->impl->get_field(slot, which, out);
+  });
+}
+int pf5d_engine_set_field(pf5d_engine* e, int slot, const double* X) {
+  return guarded([&] { e->impl->set_field(slot, X); });
+}
+void* pf5d_engine_field_ptr(pf5d_engine* e, int slot, int which) {
+  void* p = nullptr;
+  int rc = guarded([&] { p = e->impl->field_ptr(slot, which); });
+  return rc == PF5D_OK ? p : nullptr;
+}
+void* pf5d_engine_stream(pf5d_engine* e) { return e ? (void*)e->impl->stream : nullptr; }
+int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes) {
+  return guarded([&] { e->impl->stats(active, bytes); });
+}
+int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms) {
+  return guarded([&] { *avg_ms = e->impl->time_kernel(which, reps); });
+}
+int pf5d_engine_launch_info(pf5d_engine* e, long out5[5]) {
+  return guarded([&] { e->impl->launch_info(out5); });
+}
+
+}  // extern "C"

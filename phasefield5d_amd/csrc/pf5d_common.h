@@ -1,0 +1,77 @@
+// Common device helpers for the phasefield5d MI355X step library.
+//
+// Build note: the whole library is compiled with -ffp-contract=off so that the
+// FP64 kernels evaluate every product and sum separately, in the order of the
+// reference Numba loops (fastmath=False).  FP32 kernels request fused
+// multiply-adds explicitly through mad<float>.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PF5D_DEV __device__ __forceinline__
+#define PF5D_HD __host__ __device__ __forceinline__
+
+namespace pf5d {
+
+// four-component composition / potential vector, 16 B (FP32) or 32 B (FP64)
+template <typename T>
+struct alignas(4 * sizeof(T)) V4 {
+  T v[4];
+};
+
+// a*b + c: two roundings for double (reference order), one fused rounding for float
+template <typename T>
+PF5D_DEV T mad(T a, T b, T c);
+template <>
+PF5D_DEV double mad<double>(double a, double b, double c) { return a * b + c; }
+template <>
+PF5D_DEV float mad<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename T>
+PF5D_DEV V4<T> ld4(const T* p) { return *reinterpret_cast<const V4<T>*>(p); }
+template <typename T>
+PF5D_DEV void st4(T* p, const V4<T>& v) { *reinterpret_cast<V4<T>*>(p) = v; }
+
+PF5D_HD int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// non-negative IEEE values order like their bit patterns: max via integer atomics
+PF5D_DEV void atomic_max_nonneg(double* addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+PF5D_DEV void atomic_max_nonneg(float* addr, float v) {
+  atomicMax(reinterpret_cast<unsigned int*>(addr), (unsigned int)__float_as_uint(v));
+}
+
+// max that ignores NaN (the reference abs_max only replaces on v > m)
+template <typename T>
+PF5D_DEV T max_ignore_nan(T m, T v) { return (v > m) ? v : m; }
+
+// wavefront (64 lanes) max reduction by butterfly shuffles
+template <typename T>
+PF5D_DEV T wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    T o = __shfl_xor(v, off, 64);
+    v = (o > v) ? o : v;
+  }
+  return v;
+}
+PF5D_DEV double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+enum StatusBits { ST_ACTIVE = 1, ST_HARD_OOB = 2, ST_SOFT_OOB = 4, ST_DONE = 8 };
+
+// per-simulation mutable state (FP64 scalars on the device)
+struct SimState {
+  double dt, t, last_xmax;
+  double mass[4], mean[4];
+  double smax, smin, cmin, cmax, nonfinite;
+  long long steps;
+  int status;
+  int pad;
+};
+
+}  // namespace pf5d

@@ -1,0 +1,502 @@
+// Device-resident fused explicit step.
+//
+//   K1  k_stencil : interpolation + kappa*lap(X) + elastic psi -> mu, Mob (in LDS)
+//                   -> face fluxes (each face once) -> dX/dt, block max |dX/dt|
+//   DT  k_dtctl   : per simulation: max |dX/dt| -> adaptive dt (system.py:90-112),
+//                   time, step count, FP64 mass / bounds statistics, status
+//   K2  k_update  : X += dt dX/dt, simplex clamp, misfit s for the next FFT,
+//                   FP64 partial sums (mass, bounds) per work item
+//
+// Every launch walks a work list that may span many simulations of different
+// dimension and size (batched and serving modes use the same kernels).
+//
+// Canonical tile geometry: a simulation is viewed as (n0, n1, n2) with n2
+// contiguous.  3D (NX,NY,NZ) marches along x (axis 0) with a (y,z) tile;
+// 2D (NX,NY) is (1, NX, NY) with a single plane; 1D (N) is (1, 1, N).
+// Axes present in the physical problem are summed in the reference order
+// (slowest first), so the floating-point order of the Laplacian and of the
+// divergence equals the reference's (operators.py:33-81, fluxes.py:296-395).
+#pragma once
+#include "pf5d_common.h"
+#include "pf5d_flux.h"
+#include "pf5d_interp.h"
+
+namespace pf5d {
+
+constexpr int NSTAT = 9;  // sum X (4), nonfinite, max sum, min sum, min comp, max comp
+
+template <typename T>
+struct SimDev {
+  int dim;            // physical dimension 1..3
+  int n0, n1, n2;     // canonical shape
+  long ncell;
+  T inv_dx, inv_dx2;  // 1/h, 1/h^2
+  T kappa[4];         // kappa_i * kappa_value
+  T l[4];             // rank-1 elastic vector (zeros when elasticity is off)
+  T X0[4];            // reference composition for the misfit
+  int elastic;
+  int use_mean;       // misfit about the FP64 running mean (precision mode)
+  TableView<T> tab;
+  T* X;
+  T* dX;
+  const T* psi;
+  T* s;
+  double ctld, dt_cfl, lower, upper, t_end;
+  int oob_interval;   // bounds check every N steps (reference: 100)
+  double tol_soft, tol_hard;
+};
+
+struct K1Item {
+  int sim;
+  int xa, xb;   // marching range along axis 0 (3D); 0..1 otherwise
+  int y0, z0;   // tile origin on axes 1, 2
+  int ty, tz;   // tile extent (<= BY, BZ)
+};
+
+struct K2Item {
+  int sim;
+  long c0, c1;  // cell range
+};
+
+// ---------------------------------------------------------------------------
+// K1
+// ---------------------------------------------------------------------------
+template <typename T, int BY, int BZ>
+struct K1Smem {
+  static constexpr int XY = BY + 4, XZ = BZ + 4;  // X planes with halo 2
+  static constexpr int MY = BY + 2, MZ = BZ + 2;  // mu / Mob planes with halo 1
+  static constexpr size_t x_off = 0;
+  static constexpr size_t x_sz = sizeof(V4<T>) * 3 * XY * XZ;
+  static constexpr size_t mu_off = x_off + x_sz;
+  static constexpr size_t mu_sz = sizeof(V4<T>) * 2 * MY * MZ;
+  static constexpr size_t mb_off = mu_off + mu_sz;
+  static constexpr size_t mb_sz = sizeof(T) * 2 * 5 * MY * MZ;
+  static constexpr size_t fy_off = mb_off + mb_sz;
+  static constexpr size_t fy_sz = sizeof(V4<T>) * (BY + 1) * BZ;
+  static constexpr size_t fz_off = fy_off + fy_sz;
+  static constexpr size_t fz_sz = sizeof(V4<T>) * BY * (BZ + 1);
+  static constexpr size_t red_off = fz_off + fz_sz;
+  static constexpr size_t bytes = red_off + sizeof(T) * 16;
+};
+
+template <typename T, int BY, int BZ, int NT>
+__global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
+                                                T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
+                                                int nitems) {
+  using S = K1Smem<T, BY, BZ>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  V4<T>* XS = reinterpret_cast<V4<T>*>(smem + S::x_off);
+  V4<T>* MU = reinterpret_cast<V4<T>*>(smem + S::mu_off);
+  T* MB = reinterpret_cast<T*>(smem + S::mb_off);
+  V4<T>* FY = reinterpret_cast<V4<T>*>(smem + S::fy_off);
+  V4<T>* FZ = reinterpret_cast<V4<T>*>(smem + S::fz_off);
+  T* RED = reinterpret_cast<T*>(smem + S::red_off);
+  constexpr int CPT = (BY * BZ + NT - 1) / NT;
+  const int tid = threadIdx.x;
+
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const K1Item item = items[it];
+    if (!(st[item.sim].status & ST_ACTIVE)) continue;  // uniform per block
+    const SimDev<T>& sd = sims[item.sim];
+    const bool hx = sd.dim == 3, hy = sd.dim >= 2;
+    const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
+    const int ty = item.ty, tz = item.tz;
+    const T inv_dx = sd.inv_dx, inv_dx2 = sd.inv_dx2;
+    const T cdim = T(2 * sd.dim);
+    // X region rows: j in [-2, ty+1] (hy) or [0,0]; cols k in [-2, tz+1]
+    const int xj0 = hy ? -2 : 0, xrows = hy ? ty + 4 : 1, xcols = tz + 4;
+    T tmax = T(0);
+    V4<T> fprev[CPT];
+
+    auto xslot = [&](int p) { return ((p % 3) + 3) % 3; };
+    auto XAT = [&](int slot, int j, int k) -> V4<T>& { return XS[(slot * S::XY + (j + 2)) * S::XZ + (k + 2)]; };
+    auto MUAT = [&](int slot, int j, int k) -> V4<T>& { return MU[(slot * S::MY + (j + 1)) * S::MZ + (k + 1)]; };
+    auto MBAT = [&](int slot, int m, int j, int k) -> T& {
+      return MB[((slot * 5 + m) * S::MY + (j + 1)) * S::MZ + (k + 1)];
+    };
+
+    auto load_plane = [&](int p) {
+      const int gx = hx ? wrap(p, n0) : 0;
+      const int slot = xslot(p);
+      const int tot = xrows * xcols;
+      for (int idx = tid; idx < tot; idx += NT) {
+        const int j = idx / xcols + xj0, k = idx % xcols - 2;
+        const int gy = hy ? wrap(item.y0 + j, n1) : 0;
+        const int gz = wrap(item.z0 + k, n2);
+        XAT(slot, j, k) = ld4<T>(sd.X + 4 * (((long)gx * n1 + gy) * n2 + gz));
+      }
+    };
+
+    // mu, Mob of plane p on the halo-1 region (corners excluded)
+    auto potentials = [&](int p) {
+      const int s0 = xslot(p), sp = xslot(p + 1), sm = xslot(p - 1), ms = ((p % 2) + 2) % 2;
+      const int gx = hx ? wrap(p, n0) : 0;
+      const int mj0 = hy ? -1 : 0, mrows = hy ? ty + 2 : 1, mcols = tz + 2;
+      const int tot = mrows * mcols;
+      for (int idx = tid; idx < tot; idx += NT) {
+        const int j = idx / mcols + mj0, k = idx % mcols - 1;
+        const bool jh = (j < 0 || j >= ty), kh = (k < 0 || k >= tz);
+        if (jh && kh) continue;
+        const V4<T> xc = XAT(s0, j, k);
+        T xa[4], P[9];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) xa[a] = xc.v[a];
+        interp9<T>(sd.tab, xa, P);
+        T acc[4];
+        bool first = true;
+        if (hx) {
+          const V4<T> u = XAT(sp, j, k), d = XAT(sm, j, k);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a] = u.v[a] + d.v[a];
+          first = false;
+        }
+        if (hy) {
+          const V4<T> u = XAT(s0, j + 1, k), d = XAT(s0, j - 1, k);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
+          first = false;
+        }
+        {
+          const V4<T> u = XAT(s0, j, k + 1), d = XAT(s0, j, k - 1);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
+        }
+        T psi = T(0);
+        if (sd.elastic) {
+          const int gy = hy ? wrap(item.y0 + j, n1) : 0;
+          const int gz = wrap(item.z0 + k, n2);
+          psi = sd.psi[((long)gx * n1 + gy) * n2 + gz];
+        }
+        V4<T> mu;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          // (acc - 2 dim x) / h^2, then P - L kappa: for FP64 each product and sum
+          // rounds separately (negation is exact), as in simulate.py:253-255
+          const T L = mad<T>(-cdim, xa[a], acc[a]) * inv_dx2;
+          T m = mad<T>(-L, sd.kappa[a], P[a]);
+          if (sd.elastic) m = mad<T>(psi, sd.l[a], m);
+          mu.v[a] = m;
+        }
+        MUAT(ms, j, k) = mu;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) MBAT(ms, m, j, k) = P[4 + m];
+      }
+    };
+
+    // flux through the face between (sa: j,k) and (sb: jb,kb)
+    auto face = [&](int xa_s, int ma_s, int ja, int ka, int xb_s, int mb_s, int jb, int kb, T* out) {
+      const V4<T> Xa = XAT(xa_s, ja, ka), Xb = XAT(xb_s, jb, kb);
+      const V4<T> ua = MUAT(ma_s, ja, ka), ub = MUAT(mb_s, jb, kb);
+      T Ma[5], Mb[5], g[4];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) {
+        Ma[m] = MBAT(ma_s, m, ja, ka);
+        Mb[m] = MBAT(mb_s, m, jb, kb);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) g[a] = (ub.v[a] - ua.v[a]) * inv_dx;
+      face_flux<T>(Xa.v, Xb.v, Ma, Mb, g, out);
+    };
+
+    const int pbeg = hx ? item.xa - 1 : 0, pend = hx ? item.xb : 0;
+    if (hx) {
+      load_plane(pbeg - 1);
+      load_plane(pbeg);
+    } else {
+      load_plane(0);
+    }
+    for (int p = pbeg; p <= pend; ++p) {
+      if (hx) load_plane(p + 1);
+      __syncthreads();
+      potentials(p);
+      __syncthreads();
+      const int q = hx ? p - 1 : p;  // output plane of this iteration
+      const bool out_ok = hx ? (p - 1 >= item.xa) : true;
+      V4<T> fx[CPT];
+      if (hx && p >= item.xa) {
+        // x face (p-1 -> p) for interior cells
+#pragma unroll
+        for (int i = 0; i < CPT; ++i) {
+          const int c = tid + i * NT;
+          if (c < ty * tz) {
+            const int j = c / tz, k = c % tz;
+            face(xslot(p - 1), ((p - 1) % 2 + 2) % 2, j, k, xslot(p), (p % 2 + 2) % 2, j, k, fx[i].v);
+          }
+        }
+      }
+      if (out_ok) {
+        const int xs = xslot(q), ms = ((q % 2) + 2) % 2;
+        if (hy) {
+          for (int idx = tid; idx < (ty + 1) * tz; idx += NT) {
+            const int j = idx / tz, k = idx % tz;
+            face(xs, ms, j - 1, k, xs, ms, j, k, FY[j * BZ + k].v);
+          }
+        }
+        for (int idx = tid; idx < ty * (tz + 1); idx += NT) {
+          const int j = idx / (tz + 1), k = idx % (tz + 1);
+          face(xs, ms, j, k - 1, xs, ms, j, k, FZ[j * (BZ + 1) + k].v);
+        }
+        __syncthreads();
+        const int gx = hx ? wrap(q, n0) : 0;
+#pragma unroll
+        for (int i = 0; i < CPT; ++i) {
+          const int c = tid + i * NT;
+          if (c < ty * tz) {
+            const int j = c / tz, k = c % tz;
+            const V4<T> zp = FZ[j * (BZ + 1) + k + 1], zm = FZ[j * (BZ + 1) + k];
+            T acc[4];
+            bool first = true;
+            if (hx) {
+#pragma unroll
+              for (int a = 0; a < 4; ++a) acc[a] = fx[i].v[a] - fprev[i].v[a];
+              first = false;
+            }
+            if (hy) {
+              const V4<T> yp = FY[(j + 1) * BZ + k], ym = FY[j * BZ + k];
+#pragma unroll
+              for (int a = 0; a < 4; ++a) acc[a] = first ? (yp.v[a] - ym.v[a]) : ((acc[a] + yp.v[a]) - ym.v[a]);
+              first = false;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a) acc[a] = first ? (zp.v[a] - zm.v[a]) : ((acc[a] + zp.v[a]) - zm.v[a]);
+            V4<T> o;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+              o.v[a] = acc[a] * inv_dx;
+              tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
+            }
+            const int gy = hy ? item.y0 + j : 0;
+            st4<T>(sd.dX + 4 * (((long)gx * n1 + gy) * n2 + item.z0 + k), o);
+          }
+        }
+      }
+      if (hx) {
+#pragma unroll
+        for (int i = 0; i < CPT; ++i) fprev[i] = fx[i];
+      }
+      __syncthreads();
+    }
+    // block max -> one atomic per item
+    T m = wave_max<T>(tmax);
+    if ((tid & 63) == 0) RED[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      T r = RED[0];
+      for (int w = 1; w < NT / 64; ++w) r = (RED[w] > r) ? RED[w] : r;
+      atomic_max_nonneg(&xmax_acc[item.sim], r);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DT: one thread block per simulation
+//   * reduce the previous K2 partials -> mass, bounds statistics (FP64, Neumaier)
+//   * bounds check every oob_interval steps (system.py:12-53; simulate.py:231)
+//   * adaptive dt (system.py:90-112) from max |dX/dt| of this step
+// ---------------------------------------------------------------------------
+PF5D_DEV double adaptive_dt(double dt, double x, double ctld, double t_cfl, double lo_lim, double hi_lim) {
+  double raw;
+  if (!(x > 0.0)) {
+    raw = dt;
+  } else {
+    double lo = lo_lim / x, hi = hi_lim / x;
+    if (lo > hi) {
+      const double tmp = lo;
+      lo = hi;
+      hi = tmp;
+    }
+    raw = dt < lo ? lo : dt;   // numpy clip: min(max(dt, lo), hi)
+    raw = raw > hi ? hi : raw;
+  }
+  double out = dt * 2.0;
+  out = raw < out ? raw : out;
+  const double cap = 0.005 * ctld;
+  out = cap < out ? cap : out;
+  out = t_cfl < out ? t_cfl : out;
+  return out;
+}
+
+template <typename T>
+__global__ void k_dtctl(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __restrict__ xmax_acc,
+                        const double* __restrict__ part, const int* __restrict__ part_range, int nsims,
+                        int advance) {
+  const int sim = blockIdx.x;
+  if (sim >= nsims) return;
+  __shared__ double red[16][NSTAT];
+  __shared__ double comp[4];
+  const int p0 = part_range[2 * sim], p1 = part_range[2 * sim + 1];
+  // Neumaier sums of the mass partials, min/max of the rest
+  double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
+  double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
+  for (int i = p0 + threadIdx.x; i < p1; i += blockDim.x) {
+    const double* q = part + (long)i * NSTAT;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double v = q[a], t = sum[a] + v;
+      cmp[a] += (fabs(sum[a]) >= fabs(v)) ? ((sum[a] - t) + v) : ((v - t) + sum[a]);
+      sum[a] = t;
+    }
+    nf += q[4];
+    smax = fmax(smax, q[5]);
+    smin = fmin(smin, q[6]);
+    cmin = fmin(cmin, q[7]);
+    cmax = fmax(cmax, q[8]);
+  }
+  double v[NSTAT] = {sum[0] + cmp[0], sum[1] + cmp[1], sum[2] + cmp[2], sum[3] + cmp[3], nf, smax, smin, cmin, cmax};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 5; ++a) v[a] += __shfl_xor(v[a], off, 64);
+    v[5] = fmax(v[5], __shfl_xor(v[5], off, 64));
+    v[6] = fmin(v[6], __shfl_xor(v[6], off, 64));
+    v[7] = fmin(v[7], __shfl_xor(v[7], off, 64));
+    v[8] = fmax(v[8], __shfl_xor(v[8], off, 64));
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < NSTAT; ++k) red[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  (void)comp;
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+    for (int a = 0; a < 5; ++a) v[a] += red[w][a];
+    v[5] = fmax(v[5], red[w][5]);
+    v[6] = fmin(v[6], red[w][6]);
+    v[7] = fmin(v[7], red[w][7]);
+    v[8] = fmax(v[8], red[w][8]);
+  }
+  SimState& S = st[sim];
+  const SimDev<T>& sd = sims[sim];
+  const double x = (double)xmax_acc[sim];
+  xmax_acc[sim] = T(0);
+  if (!(S.status & ST_ACTIVE)) return;
+  if (p1 > p0) {
+    for (int a = 0; a < 4; ++a) {
+      S.mass[a] = v[a];
+      S.mean[a] = v[a] / (double)sd.ncell;
+    }
+    S.nonfinite = v[4];
+    S.smax = v[5];
+    S.smin = v[6];
+    S.cmin = v[7];
+    S.cmax = v[8];
+  }
+  if (!advance) return;
+  // bounds check of X_n before step n (the reference checks every 100 steps)
+  if (sd.oob_interval > 0 && (S.steps % sd.oob_interval) == 0) {
+    const double th = sd.tol_hard, ts = sd.tol_soft;
+    const bool hard = S.nonfinite > 0 || S.smax > 1.0 + th || S.smin < -th || S.cmin < -th || S.cmax > 1.0 + th;
+    const bool soft = S.smax > 1.0 + ts || S.smin < -ts || S.cmin < -ts || S.cmax > 1.0 + ts;
+    if (soft) S.status |= ST_SOFT_OOB;
+    if (hard) {
+      S.status = (S.status & ~ST_ACTIVE) | ST_HARD_OOB;
+      return;
+    }
+  }
+  if (S.t >= sd.t_end) {
+    S.status = (S.status & ~ST_ACTIVE) | ST_DONE;
+    return;
+  }
+  const double dt = adaptive_dt(S.dt, x, sd.ctld, sd.dt_cfl, sd.lower, sd.upper);
+  S.dt = dt;
+  S.t = S.t + dt;
+  S.last_xmax = x;
+  S.steps += 1;
+}
+
+// ---------------------------------------------------------------------------
+// K2
+// ---------------------------------------------------------------------------
+template <typename T>
+PF5D_DEV void neumaier_add(double& s, double& c, double v) {
+  const double t = s + v;
+  c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+  s = t;
+}
+
+// mode 0: update (X += dt dX, clamp, misfit, partials); mode 1: prime (misfit and
+// partials of the current X only)
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st,
+                                               const K2Item* __restrict__ items, int nitems, double* __restrict__ part,
+                                               int prime) {
+  __shared__ double red[NT / 64][NSTAT];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const K2Item item = items[it];
+    const SimState& S = st[item.sim];
+    const bool run = prime || (S.status & ST_ACTIVE);
+    const SimDev<T>& sd = sims[item.sim];
+    const double dt = S.dt;
+    double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
+    double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
+    T ref[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) ref[a] = sd.use_mean ? (T)S.mean[a] : sd.X0[a];
+    for (long c = item.c0 + threadIdx.x; c < item.c1; c += NT) {
+      V4<T> x = ld4<T>(sd.X + 4 * c);
+      if (run && !prime) {
+        const V4<T> d = ld4<T>(sd.dX + 4 * c);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          if (sizeof(T) == 8)
+            x.v[a] = x.v[a] + (T)((double)d.v[a] * dt);
+          else
+            x.v[a] = (T)((double)x.v[a] + (double)d.v[a] * dt);
+        }
+        const T tot = ((x.v[0] + x.v[1]) + x.v[2]) + x.v[3];
+        if (tot > T(1)) {
+          const T inv = T(1) / tot;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) x.v[a] = x.v[a] * inv;
+        }
+        st4<T>(sd.X + 4 * c, x);
+      }
+      if (sd.elastic && run) {
+        T acc = (x.v[0] - ref[0]) * sd.l[0];
+#pragma unroll
+        for (int a = 1; a < 4; ++a) acc = mad<T>(x.v[a] - ref[a], sd.l[a], acc);
+        sd.s[c] = acc;
+      }
+      double csum = 0;
+      bool fin = true;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const double v = (double)x.v[a];
+        neumaier_add<T>(sum[a], cmp[a], v);
+        fin = fin && isfinite(v);
+        csum += v;
+        cmin = fmin(cmin, v);
+        cmax = fmax(cmax, v);
+      }
+      if (!fin) nf += 1;
+      smax = fmax(smax, csum);
+      smin = fmin(smin, csum);
+    }
+    double v[NSTAT] = {sum[0] + cmp[0], sum[1] + cmp[1], sum[2] + cmp[2], sum[3] + cmp[3], nf, smax, smin, cmin, cmax};
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int a = 0; a < 5; ++a) v[a] += __shfl_xor(v[a], off, 64);
+      v[5] = fmax(v[5], __shfl_xor(v[5], off, 64));
+      v[6] = fmin(v[6], __shfl_xor(v[6], off, 64));
+      v[7] = fmin(v[7], __shfl_xor(v[7], off, 64));
+      v[8] = fmax(v[8], __shfl_xor(v[8], off, 64));
+    }
+    if ((threadIdx.x & 63) == 0)
+      for (int k = 0; k < NSTAT; ++k) red[threadIdx.x >> 6][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < NT / 64; ++w) {
+        for (int a = 0; a < 5; ++a) v[a] += red[w][a];
+        v[5] = fmax(v[5], red[w][5]);
+        v[6] = fmin(v[6], red[w][6]);
+        v[7] = fmin(v[7], red[w][7]);
+        v[8] = fmax(v[8], red[w][8]);
+      }
+      for (int k = 0; k < NSTAT; ++k) part[(long)it * NSTAT + k] = v[k];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace pf5d

@@ -1,0 +1,172 @@
+// 4D CALPHAD multilinear interpolation of the nine properties
+// [mu_Mn, mu_Ni, mu_Co, mu_Cu, Mob_Fe, Mob_Mn, Mob_Ni, Mob_Co, Mob_Cu].
+//
+// Follows phasefield5d/solver/interpolation.py:67-147 and :154-211:
+//   * linear region: sum(X) <= 1 - 4h - 1e-9 (interpolation.py:195);
+//   * per axis u = X/h; u <= 0 -> (i=0, t=0); u >= num-1 -> (i=num-2, t=1);
+//     else i = int(u), t = u - i (interpolation.py:110-121);
+//   * 16 weights prod(t or 1-t), summed c0000..c1111 with i3 fastest
+//     (interpolation.py:126-147), left to right;
+//   * outside the linear region the value of the nearest table node in
+//     Euclidean X (the reference's cKDTree query, interpolation.py:203-209).
+//     For a table whose rows are the simplex grid nodes, the nearest node to a
+//     point with X >= 0, sum(X) <= 1 lies within [floor(u)-1, floor(u)+2] on
+//     every axis (the floor node is closer than 2h), so that 4^4 window is
+//     searched exhaustively; ties keep the lowest row-major node.
+//
+// Device table layout: dense (num^4, NPAD) with the nine properties first and
+// NPAD = 10 (FP64, five 16-byte loads per corner) or 12 (FP32, three 16-byte
+// loads per corner).  Nodes outside the simplex hold NaN, as in build_4d_grid.
+#pragma once
+#include "pf5d_common.h"
+
+namespace pf5d {
+
+template <typename T>
+struct TablePad;
+template <>
+struct TablePad<double> { static constexpr int value = 10; };
+template <>
+struct TablePad<float> { static constexpr int value = 12; };
+
+template <typename T>
+struct TableView {
+  const T* data;   // (num^4, NPAD)
+  int num;
+  T inv_step;      // 1/h (host: 1.0/h in FP64, rounded for FP32)
+  T thresh;        // 1 - 4h - 1e-9
+  double step;     // h, for node coordinates in the nearest search
+};
+
+template <typename T>
+PF5D_DEV void load_corner(const T* p, T g[TablePad<T>::value]);
+
+template <>
+PF5D_DEV void load_corner<double>(const double* p, double g[10]) {
+  const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    double2 v = q[i];
+    g[2 * i] = v.x;
+    g[2 * i + 1] = v.y;
+  }
+}
+template <>
+PF5D_DEV void load_corner<float>(const float* p, float g[12]) {
+  const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float4 v = q[i];
+    g[4 * i] = v.x;
+    g[4 * i + 1] = v.y;
+    g[4 * i + 2] = v.z;
+    g[4 * i + 3] = v.w;
+  }
+}
+
+template <typename T>
+PF5D_DEV void axis_index(T u, int num, int& i, T& t) {
+  if (u <= T(0)) {
+    i = 0;
+    t = T(0);
+  } else if (u >= T(num - 1)) {
+    i = num - 2;
+    t = T(1);
+  } else {
+    i = (int)u;
+    t = u - (T)i;
+  }
+}
+
+template <typename T>
+PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  const int num = tv.num;
+  int i0, i1, i2, i3;
+  T t0, t1, t2, t3;
+  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
+  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
+  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
+  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
+  const T s0 = T(1) - t0, s1 = T(1) - t1, s2 = T(1) - t2, s3 = T(1) - t3;
+  const long n3 = (long)num * num * num, n2 = (long)num * num, n1 = num;
+  const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
+#pragma unroll
+  for (int p = 0; p < 9; ++p) P[p] = T(0);
+#pragma unroll
+  for (int b = 0; b < 16; ++b) {
+    const int b0 = (b >> 3) & 1, b1 = (b >> 2) & 1, b2 = (b >> 1) & 1, b3 = b & 1;
+    T w = (b0 ? t0 : s0) * (b1 ? t1 : s1);
+    w = w * (b2 ? t2 : s2);
+    w = w * (b3 ? t3 : s3);
+    T g[NP];
+    load_corner<T>(base + (b0 * n3 + b1 * n2 + b2 * n1 + b3) * NP, g);
+    if (b == 0) {
+#pragma unroll
+      for (int p = 0; p < 9; ++p) P[p] = w * g[p];
+    } else {
+#pragma unroll
+      for (int p = 0; p < 9; ++p) P[p] = mad<T>(w, g[p], P[p]);
+    }
+  }
+}
+
+template <typename T>
+PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  const int num = tv.num;
+  const double h = tv.step;
+  int lo[4];
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    double u = (double)X[d] / h;
+    int f = (u <= 0.0) ? 0 : (int)u;
+    if (f > num - 1) f = num - 1;
+    lo[d] = f - 1;
+  }
+  double best = 1e300;
+  long bidx = -1;
+  for (int a = 0; a < 4; ++a) {
+    int n0 = lo[0] + a;
+    if (n0 < 0 || n0 >= num) continue;
+    for (int b = 0; b < 4; ++b) {
+      int n1 = lo[1] + b;
+      if (n1 < 0 || n1 >= num || n0 + n1 > num - 1) continue;
+      for (int c = 0; c < 4; ++c) {
+        int n2 = lo[2] + c;
+        if (n2 < 0 || n2 >= num || n0 + n1 + n2 > num - 1) continue;
+        for (int e = 0; e < 4; ++e) {
+          int n3 = lo[3] + e;
+          if (n3 < 0 || n3 >= num || n0 + n1 + n2 + n3 > num - 1) continue;
+          double d0 = (double)X[0] - n0 * h, d1 = (double)X[1] - n1 * h;
+          double d2 = (double)X[2] - n2 * h, d3 = (double)X[3] - n3 * h;
+          double dd = d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
+          if (dd < best) {
+            best = dd;
+            bidx = (((long)n0 * num + n1) * num + n2) * num + n3;
+          }
+        }
+      }
+    }
+  }
+  if (bidx < 0) {
+#pragma unroll
+    for (int p = 0; p < 9; ++p) P[p] = T(__builtin_nan(""));
+    return;
+  }
+  T g[NP];
+  load_corner<T>(tv.data + bidx * NP, g);
+#pragma unroll
+  for (int p = 0; p < 9; ++p) P[p] = g[p];
+}
+
+template <typename T>
+PF5D_DEV void interp9(const TableView<T>& tv, const T X[4], T P[9]) {
+  const T s = ((X[0] + X[1]) + X[2]) + X[3];
+  if (s <= tv.thresh)
+    interp_linear<T>(tv, X, P);
+  else
+    interp_nearest<T>(tv, X, P);
+}
+
+}  // namespace pf5d

@@ -1,0 +1,215 @@
+"""Device parity against the FP64 oracle (oracle/pf5d_oracle.py), through the C ABI.
+
+Bars: per-function FP64 kernels without FFT are bit-exact (same operation
+order, no contraction); anything involving the FFT pair is held to 1e-12
+normwise relative (north-star FP64 tolerance); FP32 results to 1e-5 absolute in
+composition after N steps.
+"""
+import numpy as np
+import pytest
+
+from helpers import developed_state, oracle_sim, problem, rel_inf
+from oracle import pf5d_oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from phasefield5d_amd import _lib
+    _lib.require_gpu()
+    return _lib
+
+
+def test_device_info(dev):
+    import ctypes as C
+    out = (C.c_int * 3)()
+    dev.check(dev.lib().pf5d_device_info(out))
+    print("CUs, LDS/block, warp:", list(out))
+    assert out[0] > 0 and out[2] == 64
+
+
+@pytest.mark.parametrize("dim,n", [(1, 160), (2, 32), (3, 16)])
+def test_interpolator_bitwise(dev, dim, n):
+    from phasefield5d_amd.solver.interpolation import interpolator_nb
+    pb = problem(dim, n)
+    X = developed_state(pb, amp=0.08)
+    # cells past the linear region (nearest-node fallback) and a negative component
+    Xf = X.reshape(-1, 4)
+    Xf[3] = [0.3, 0.3, 0.2, 0.15]
+    Xf[5] = [0.33, 0.31, 0.2, 0.16]
+    Xf[7] = [-0.002, 0.2, 0.3, 0.2]
+    ref = O.interpolator(X, pb.data_grid, pb.cfg.resolution, pb.tree, pb.calphad_values)
+    out = interpolator_nb(X, pb.data_grid, pb.cfg.resolution, pb.tree, pb.calphad_values)
+    assert np.array_equal(out, ref)
+    # device tensors in and out, strided output view
+    Xd = torch.from_numpy(X).cuda()
+    P = torch.empty(X.shape[:-1] + (9,), dtype=torch.float64, device="cuda")
+    interpolator_nb(Xd, pb.data_grid, pb.cfg.resolution, pb.tree, pb.calphad_values, out=P)
+    assert np.array_equal(P.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("dim,n", [(1, 160), (2, 48), (3, 24)])
+def test_laplacian_and_divergence_bitwise(dev, dim, n):
+    from phasefield5d_amd.solver.fluxes import get_divergence_from_mu_function
+    from phasefield5d_amd.solver.operators import calculate_laplacian
+    pb = problem(dim, n)
+    X = developed_state(pb)
+    assert np.array_equal(calculate_laplacian(X, pb.dx), O.laplacian(X, pb.dx))
+    P = O.interpolator(X, pb.data_grid, pb.cfg.resolution, pb.tree, pb.calphad_values)
+    Pd = torch.from_numpy(P).cuda()
+    Xd = torch.from_numpy(X).cuda()
+    out = torch.empty_like(Xd)
+    get_divergence_from_mu_function(dim)(Xd, Pd[..., 4:], Pd[..., :4], 1.0 / pb.dx, out)
+    ref = O.divergence_from_mu(X, P[..., 4:], P[..., :4], 1.0 / pb.dx)
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_scalar_kernels(dev):
+    from phasefield5d_amd.solver.operators import abs_max, clamp_simplex, update_composition
+    from phasefield5d_amd.solver.system import out_of_bounds
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(17, 31, 4))
+    a[2, 3, 1] = np.nan
+    assert abs_max(a) == O.abs_max(a)
+    assert abs_max(np.zeros((5, 4))) == 0.0
+    X = rng.uniform(0, 0.4, size=(1000, 4))
+    ref = O.clamp_simplex(X.copy())
+    got = clamp_simplex(X.copy())
+    assert np.array_equal(got, ref)
+    dX = rng.normal(size=X.shape)
+    Xu, dXu = X.copy(), dX.copy()
+    update_composition(Xu, dXu, 0.37)
+    assert np.array_equal(dXu, dX * 0.37) and np.array_equal(Xu, X + dX * 0.37)
+    assert not out_of_bounds(np.full((16, 4), 0.1))
+    bad = np.full((16, 4), 0.1)
+    bad[5, 2] = np.nan
+    assert out_of_bounds(bad)
+    bad = np.full((16, 4), 0.1)
+    bad[3, 0] = 1.5
+    assert out_of_bounds(bad)
+
+
+@pytest.mark.parametrize("dim,shape", [(1, (160,)), (1, (189,)), (2, (48, 96)), (2, (64, 64)), (3, (16, 24, 32)),
+                                       (3, (12, 12, 12))])
+def test_elastic_update(dev, dim, shape):
+    from phasefield5d_amd.solver.elastic import make_elastic_updater
+    from phasefield5d_amd.solver.system import (build_fourier_grid_1d_along_direction,
+                                                build_fourier_grid_2d_in_plane, build_fourier_grid_3d)
+    from phasefield5d_amd.solver.elastic import build_khachaturyan_kernel
+    dx = 1e-10
+    if dim == 1:
+        kg, kn = build_fourier_grid_1d_along_direction(shape[0], dx, [1, 1, 0])
+    elif dim == 2:
+        kg, kn = build_fourier_grid_2d_in_plane(shape[0], shape[1], dx, [0, 1, 1])
+    else:
+        kg, kn = build_fourier_grid_3d(*shape, dx)
+    B = build_khachaturyan_kernel(kg, kn, 240.0, 150.0, 120.0)
+    rng = np.random.default_rng(0)
+    lih = np.array([0.086, 0.004, 0.008, 0.029])
+    Lam = np.outer(lih, lih) * 7.1e-6
+
+    class Cfg:
+        include_cubic_anisotropy = True
+        initial_composition = np.array([0.1, 0.2, 0.2, 0.3])
+
+    X = Cfg.initial_composition + 1e-2 * rng.normal(size=shape + (4,))
+    upd = make_elastic_updater(Cfg(), Lam, B, X.shape, np.float64)
+    mu0 = rng.normal(size=shape + (4,)) * 1e3
+    mu = mu0.copy()
+    upd(mu, X)
+    from phasefield5d_amd.solver.elastic import rank_one_vector
+    l, _ = rank_one_vector(Lam)
+    ref = O.elastic_update(mu0.copy(), X, Cfg.initial_composition, l, B)
+    assert rel_inf(mu - mu0, ref - mu0) < TOL64
+
+
+def _engine_one_step(pb, precision="fp64"):
+    from phasefield5d_amd.engine import Engine
+    eng = Engine(precision)
+    slot = eng.add_problem(pb)
+    eng.step(1)
+    eng.sync()
+    return eng, slot
+
+
+@pytest.mark.parametrize("dim,n,elastic", [(1, 160, True), (2, 32, True), (2, 32, False), (3, 16, True),
+                                           (3, 24, False)])
+def test_fused_step_fp64(dev, dim, n, elastic):
+    pb = problem(dim, n, elastic=elastic)
+    pb.X_init = developed_state(pb)
+    ora = oracle_sim(pb)
+    Xn, dX, dt_new, _, xmax = ora.step(pb.X_init, pb.dt0)
+    eng, slot = _engine_one_step(pb)
+    st = eng.state(slot)
+    dXg = eng.field(slot, "dXdt")
+    if elastic:
+        assert rel_inf(dXg, dX) < TOL64
+        assert abs(st["dt"] - dt_new) <= TOL64 * dt_new
+        assert rel_inf(eng.field(slot, "X") - pb.X_init, Xn - pb.X_init) < 1e-10
+    else:
+        assert np.array_equal(dXg, dX)
+        assert st["dt"] == dt_new and st["last_xmax"] == xmax
+        assert np.array_equal(eng.field(slot, "X"), Xn)
+    assert st["steps"] == 1 and st["t"] == pytest.approx(dt_new, rel=TOL64)
+
+
+def test_fused_trajectory_per_step_fp64(dev):
+    """20 device steps; each compared with one oracle step from the device's own state."""
+    from phasefield5d_amd.engine import Engine
+    pb = problem(2, 32)
+    pb.X_init = developed_state(pb, amp=0.04)
+    ora = oracle_sim(pb)
+    eng = Engine("fp64")
+    slot = eng.add_problem(pb)
+    X, dt = pb.X_init.copy(), pb.dt0
+    worst = 0.0
+    for _ in range(20):
+        Xn, dX, dt_new, _, _ = ora.step(X, dt)
+        eng.step(1)
+        st = eng.state(slot)
+        worst = max(worst, rel_inf(eng.field(slot, "dXdt"), dX), abs(st["dt"] - dt_new) / dt_new)
+        X, dt = eng.field(slot, "X"), st["dt"]
+    assert worst < TOL64
+
+
+def test_fp32_engine_error(dev):
+    """FP32 step vs the FP64 oracle trajectory: max |dX| composition error after 50 steps."""
+    from phasefield5d_amd.engine import Engine
+    pb = problem(2, 32)
+    pb.X_init = developed_state(pb, amp=0.04)
+    ora = oracle_sim(pb)
+    X, dt = pb.X_init.copy(), pb.dt0
+    for _ in range(50):
+        X, _, dt, _, _ = ora.step(X, dt)
+    eng = Engine("fp32")
+    slot = eng.add_problem(pb)
+    eng.step(50)
+    Xg = eng.field(slot, "X")
+    err = np.abs(Xg - X).max()
+    print("fp32 max |X - X_fp64| after 50 steps:", err)
+    assert err < 1e-5
+    st = eng.state(slot)
+    assert np.allclose(st["mass"], X.reshape(-1, 4).sum(axis=0), rtol=1e-9)
+
+
+def test_batched_equals_alone(dev):
+    """Each simulation's result is bit-identical alone and inside a mixed batch."""
+    from phasefield5d_amd.engine import Engine
+    pbs = [problem(1, 160), problem(2, 32, seed=2), problem(3, 16, seed=3), problem(2, 48, seed=4, elastic=False)]
+    for p in pbs:
+        p.X_init = developed_state(p, amp=0.03)
+    alone = []
+    for p in pbs:
+        e = Engine("fp64")
+        s = e.add_problem(p)
+        e.step(7)
+        alone.append(e.field(s, "X"))
+    mix = Engine("fp64")
+    slots = [mix.add_problem(p) for p in pbs]
+    mix.step(7)
+    for s, ref in zip(slots, alone):
+        assert np.array_equal(mix.field(s, "X"), ref)
