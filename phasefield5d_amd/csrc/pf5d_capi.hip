@@ -530,7 +530,7 @@ struct Engine : EngineBase {
     bool used = false;
     pf5d_sim_desc d{};
     long ncell = 0;
-    T *X = nullptr, *dX = nullptr, *psi = nullptr, *s = nullptr, *bhat = nullptr;
+    T *X = nullptr, *Xlo = nullptr, *dX = nullptr, *psi = nullptr, *s = nullptr, *bhat = nullptr;
     cx<T>* C = nullptr;
     const pf5d_table* tab = nullptr;
     size_t bytes = 0;
@@ -583,6 +583,7 @@ struct Engine : EngineBase {
   }
   void free_sim(Sim& s) {
     dfree(s.X);
+    dfree(s.Xlo);
     dfree(s.dX);
     dfree(s.psi);
     dfree(s.s);
@@ -644,6 +645,10 @@ struct Engine : EngineBase {
       s.X = (T*)dmalloc(ncell * 4 * sizeof(T));
       s.dX = (T*)dmalloc(ncell * 4 * sizeof(T));
       s.bytes = ncell * 8 * sizeof(T);
+      if (sizeof(T) == 4) {
+        s.Xlo = (T*)dmalloc(ncell * 4 * sizeof(T));
+        s.bytes += ncell * 4 * sizeof(T);
+      }
       if (d->elastic) {
         s.psi = (T*)dmalloc(ncell * sizeof(T));
         s.s = (T*)dmalloc(ncell * sizeof(T));
@@ -665,10 +670,15 @@ struct Engine : EngineBase {
     *slot = id;
     dirty = true;
   }
+  // FP32 mode keeps X as the nearest FP32 value plus an FP32 remainder
   void upload_X(Sim& s, const double* Xh) {
-    std::vector<T> h(s.ncell * 4);
-    for (long i = 0; i < s.ncell * 4; ++i) h[i] = (T)Xh[i];
+    std::vector<T> h(s.ncell * 4), lo(s.Xlo ? s.ncell * 4 : 0);
+    for (long i = 0; i < s.ncell * 4; ++i) {
+      h[i] = (T)Xh[i];
+      if (s.Xlo) lo[i] = (T)(Xh[i] - (double)h[i]);
+    }
     HIPCHK(hipMemcpy(s.X, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (s.Xlo) HIPCHK(hipMemcpy(s.Xlo, lo.data(), lo.size() * sizeof(T), hipMemcpyHostToDevice));
   }
   Sim& at(int slot) {
     require(slot >= 0 && slot < (int)sims.size() && sims[slot].used, "invalid slot " + std::to_string(slot));
@@ -711,6 +721,7 @@ struct Engine : EngineBase {
     v.use_mean = (sizeof(T) == 4) ? 1 : 0;
     v.tab = view<T>(s.tab);
     v.X = s.X;
+    v.Xlo = s.Xlo;
     v.dX = s.dX;
     v.psi = s.psi;
     v.s = s.s;
@@ -875,10 +886,11 @@ struct Engine : EngineBase {
     const T* src = which == 0 ? s.X : which == 1 ? s.dX : s.psi;
     require(src != nullptr, "field not present");
     const long n = (which == 2) ? s.ncell : s.ncell * 4;
-    std::vector<T> h(n);
+    std::vector<T> h(n), lo((which == 0 && s.Xlo) ? n : 0);
     HIPCHK(hipMemcpyAsync(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    if (!lo.empty()) HIPCHK(hipMemcpyAsync(lo.data(), s.Xlo, n * sizeof(T), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    for (long i = 0; i < n; ++i) out[i] = (double)h[i];
+    for (long i = 0; i < n; ++i) out[i] = lo.empty() ? (double)h[i] : (double)h[i] + (double)lo[i];
   }
   void set_field(int slot, const double* X) override {
     Sim& s = at(slot);

@@ -38,6 +38,7 @@ struct SimDev {
   int use_mean;       // misfit about the FP64 running mean (precision mode)
   TableView<T> tab;
   T* X;
+  T* Xlo;             // FP32 mode: rounding remainder, the state is X + Xlo (null in FP64 mode)
   T* dX;
   const T* psi;
   T* s;
@@ -434,34 +435,63 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
     for (int a = 0; a < 4; ++a) ref[a] = sd.use_mean ? (T)S.mean[a] : sd.X0[a];
     for (long c = item.c0 + threadIdx.x; c < item.c1; c += NT) {
       V4<T> x = ld4<T>(sd.X + 4 * c);
+      double xd[4];  // the state in FP64 (FP32 mode: X + Xlo)
+      if (sd.Xlo) {
+        const V4<T> lo = ld4<T>(sd.Xlo + 4 * c);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a] + (double)lo.v[a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
+      }
       if (run && !prime) {
         const V4<T> d = ld4<T>(sd.dX + 4 * c);
+        if (!sd.Xlo) {
+          // FP64: composition_change *= dt; X += composition_change; clamp (simulate.py:316-320)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          if (sizeof(T) == 8)
-            x.v[a] = x.v[a] + (T)((double)d.v[a] * dt);
-          else
-            x.v[a] = (T)((double)x.v[a] + (double)d.v[a] * dt);
-        }
-        const T tot = ((x.v[0] + x.v[1]) + x.v[2]) + x.v[3];
-        if (tot > T(1)) {
-          const T inv = T(1) / tot;
+          for (int a = 0; a < 4; ++a) x.v[a] = x.v[a] + (T)((double)d.v[a] * dt);
+          const T tot = ((x.v[0] + x.v[1]) + x.v[2]) + x.v[3];
+          if (tot > T(1)) {
+            const T inv = T(1) / tot;
 #pragma unroll
-          for (int a = 0; a < 4; ++a) x.v[a] = x.v[a] * inv;
+            for (int a = 0; a < 4; ++a) x.v[a] = x.v[a] * inv;
+          }
+#pragma unroll
+          for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
+          st4<T>(sd.X + 4 * c, x);
+        } else {
+          // FP32 state with an FP32 remainder: update and clamp in FP64, then split
+          // into the nearest FP32 value and what it leaves out, so increments below
+          // half an FP32 ulp accumulate instead of being rounded away
+#pragma unroll
+          for (int a = 0; a < 4; ++a) xd[a] = xd[a] + (double)d.v[a] * dt;
+          const double tot = ((xd[0] + xd[1]) + xd[2]) + xd[3];
+          if (tot > 1.0) {
+            const double inv = 1.0 / tot;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) xd[a] = xd[a] * inv;
+          }
+          V4<T> lo;
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            x.v[a] = (T)xd[a];
+            lo.v[a] = (T)(xd[a] - (double)x.v[a]);
+          }
+          st4<T>(sd.X + 4 * c, x);
+          st4<T>(sd.Xlo + 4 * c, lo);
         }
-        st4<T>(sd.X + 4 * c, x);
       }
       if (sd.elastic && run) {
-        T acc = (x.v[0] - ref[0]) * sd.l[0];
+        T acc = (T)(xd[0] - (double)ref[0]) * sd.l[0];
 #pragma unroll
-        for (int a = 1; a < 4; ++a) acc = mad<T>(x.v[a] - ref[a], sd.l[a], acc);
+        for (int a = 1; a < 4; ++a) acc = mad<T>((T)(xd[a] - (double)ref[a]), sd.l[a], acc);
         sd.s[c] = acc;
       }
       double csum = 0;
       bool fin = true;
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
-        const double v = (double)x.v[a];
+        const double v = xd[a];
         neumaier_add<T>(sum[a], cmp[a], v);
         fin = fin && isfinite(v);
         csum += v;

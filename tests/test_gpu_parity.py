@@ -193,7 +193,11 @@ def test_fp32_engine_error(dev):
     print("fp32 max |X - X_fp64| after 50 steps:", err)
     assert err < 1e-5
     st = eng.state(slot)
-    assert np.allclose(st["mass"], X.reshape(-1, 4).sum(axis=0), rtol=1e-9)
+    # the FP32 state carries its rounding remainder, so the FP64 mass is held
+    m0 = pb.X_init.reshape(-1, 4).sum(axis=0)
+    drift = np.abs(st["mass"] - m0) / m0
+    print("fp32 relative mass drift after 50 steps:", drift)
+    assert drift.max() < 1e-12
 
 
 def test_batched_equals_alone(dev):
