@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--variant", type=int, default=None)
     ap.add_argument("--cx", type=int, default=None)
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="end with a 256 MiB device-to-device copy (scales FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--fp32-only", action="store_true", help="only the FP32 fused leg (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the explicit step of one simulation "
@@ -90,6 +93,10 @@ def main():
         slot = eng.add_problem(pb)
         if args.variant is not None or args.cx is not None:
             eng.set_tiling(args.variant or 0, args.cx or 16)
+        else:
+            from phasefield5d_amd import autotune
+            tuned = autotune.tune(eng, chunks=(16, 32), reps=5)
+            log(f"{precision} autotuned: {tuned}")
         eng.step(warmup)
         eng.sync()
         torch.cuda.synchronize()
@@ -110,33 +117,50 @@ def main():
     log(f"fp32: {v32:.4g} cell-updates/s, {1e3 * el32 / args.steps:.3f} ms/step; K1 {k1_ms:.3f} ms, FFT {fft_ms:.3f} ms",
         info)
     st32 = e32.state(s32)
+    if not (st32["status"] & 1) or st32["steps"] != args.warmup + args.steps:
+        raise RuntimeError(f"fp32 simulation stopped early: {st32}")
 
-    # FP64 fused, same number of steps from the same state (error reference)
-    log("fp64 fused")
-    e64, s64, el64 = run_engine("fp64", args.steps, args.warmup)
-    v64 = ncell * args.steps / el64
-    k1_64 = e64.time_kernel("k1", reps=10)
-    X32, X64 = e32.field(s32, "X"), e64.field(s64, "X")
-    diff = np.abs(X32 - X64)
-    st64 = e64.state(s64)
     mass0 = pb.X_init.reshape(-1, 4).sum(axis=0)
     drift32 = np.abs(st32["mass"] - mass0) / mass0
-    log(f"fp64 fused: {v64:.4g} cell-updates/s; max|dX| {diff.max():.3e} rms {np.sqrt((diff**2).mean()):.3e}")
-    del e64
+    v64 = k1_64 = el64 = vplain = None
+    diff = np.zeros(1)
+    if not args.fp32_only:
+        # FP64 fused, same number of steps from the same state (error reference)
+        log("fp64 fused")
+        e64, s64, el64 = run_engine("fp64", args.steps, args.warmup)
+        v64 = ncell * args.steps / el64
+        k1_64 = e64.time_kernel("k1", reps=10)
+        X32, X64 = e32.field(s32, "X"), e64.field(s64, "X")
+        diff = np.abs(X32 - X64)
+        log(f"fp64 fused: {v64:.4g} cell-updates/s; max|dX| {diff.max():.3e} rms {np.sqrt((diff**2).mean()):.3e}")
+        del e64
 
-    # plain FP64 drop-in path
-    log("fp64 plain drop-in")
-    ps = PlainStepper(pb)
-    ps.step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.plain_steps):
+        # plain FP64 drop-in path
+        log("fp64 plain drop-in")
+        ps = PlainStepper(pb)
         ps.step()
-    torch.cuda.synchronize()
-    elp = time.perf_counter() - t0
-    vplain = ncell * args.plain_steps / elp
-    log(f"fp64 plain: {vplain:.4g} cell-updates/s")
-    del ps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            ps.step()
+        torch.cuda.synchronize()
+        elp = time.perf_counter() - t0
+        vplain = ncell * args.plain_steps / elp
+        log(f"fp64 plain: {vplain:.4g} cell-updates/s")
+        del ps
+
+    if args.calibration_copy:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_stencil_3d128_fp32_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
 
     k1_bytes = 36 * ncell
     achieved = k1_bytes / (k1_ms * 1e-3) / 1e9
@@ -148,12 +172,12 @@ def main():
                                "elasticity on, synthetic regular-solution table",
                    "cells": ncell, "shape": list(pb.shape), "precision": "fp32", "elastic": True},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_stencil<float> (fused interpolation + kappa lap + mu + face flux + divergence)",
                      "bytes_per_cell": 36, "avg_ms": k1_ms},
-        "fp64_fused": {"value": v64, "ms_per_step": 1e3 * el64 / args.steps, "k1_ms": k1_64},
+        "fp64_fused": {"value": v64, "ms_per_step": 1e3 * el64 / args.steps if el64 else None, "k1_ms": k1_64},
         "fp64_plain": {"value": vplain, "steps": args.plain_steps},
-        "ratio_fp32_vs_plain_fp64": v32 / vplain,
+        "ratio_fp32_vs_plain_fp64": v32 / vplain if vplain else None,
         "error_vs_fp64": {"max": float(diff.max()), "rms": float(np.sqrt((diff ** 2).mean())),
                           "steps": args.warmup + args.steps},
         "mass_drift_fp32": drift32.tolist(),

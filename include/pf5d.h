@@ -115,6 +115,9 @@ int pf5d_engine_remove(pf5d_engine* e, int slot);
 /* K1 tile: variant index (pf5d_engine_num_variants), x chunk (3D), per dimension */
 int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* by_bz_nt /* 3*n ints, may be NULL */);
 int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx);
+/* FFT passes: LDS bytes per work item in KiB (sets how many lines one block
+ * transforms) and threads per block (128 or 256) */
+int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt);
 /* enqueue nsteps (graph replay); does not block */
 int pf5d_engine_step(pf5d_engine* e, int nsteps);
 int pf5d_engine_sync(pf5d_engine* e);

@@ -247,9 +247,6 @@ struct FftPlans {
   }
 };
 
-constexpr int FFT_NT = 256;
-constexpr size_t FFT_LDS_BUDGET = 64 * 1024;
-
 // the five stages of psi = irfftn(Bhat rfftn(s)) and their work items
 template <typename T>
 struct FftBatch {
@@ -260,9 +257,12 @@ struct FftBatch {
   size_t lds[5] = {0, 0, 0, 0, 0};
   FftItem* d_items = nullptr;
   int offs[6] = {0, 0, 0, 0, 0, 0};
+  // tunables: LDS bytes per work item (sets lines per item) and threads per block
+  size_t budget = 16 * 1024;
+  int nt = 128;
 
-  static int pick_g(int n, long nlines) {
-    long g = (long)(FFT_LDS_BUDGET / (2 * (size_t)n * sizeof(cx<T>)));
+  int pick_g(int n, long nlines) const {
+    long g = (long)(budget / (2 * (size_t)n * sizeof(cx<T>)));
     g = std::max<long>(1, std::min<long>(g, 64));
     return (int)std::min<long>(g, nlines);
   }
@@ -387,9 +387,13 @@ struct FftBatch {
       const int n = offs[i + 1] - offs[i];
       if (n == 0) continue;
       require(lds[i] <= (size_t)di.lds, "FFT line too long for LDS: " + std::to_string(lds[i]) + " bytes");
-      const unsigned grid = (unsigned)std::min<long>(n, (long)di.cus * 8);
-      hipLaunchKernelGGL((k_fft<T, FFT_NT>), dim3(grid), dim3(FFT_NT), lds[i], stream, d_items + offs[i], n,
-                         plans.d_plans, plans.d_tw, plans.d_roots, st);
+      const unsigned grid = (unsigned)std::min<long>(n, (long)di.cus * 16);
+      if (nt == 256)
+        hipLaunchKernelGGL((k_fft<T, 256>), dim3(grid), dim3(256), lds[i], stream, d_items + offs[i], n,
+                           plans.d_plans, plans.d_tw, plans.d_roots, st);
+      else
+        hipLaunchKernelGGL((k_fft<T, 128>), dim3(grid), dim3(128), lds[i], stream, d_items + offs[i], n,
+                           plans.d_plans, plans.d_tw, plans.d_roots, st);
       HIPCHK(hipGetLastError());
     }
   }
@@ -484,6 +488,7 @@ struct EngineBase {
   virtual void* field_ptr(int slot, int which) = 0;
   virtual void stats(long* active, long* bytes) = 0;
   virtual void set_tiling(int variant, int cx) = 0;
+  virtual void set_fft(int budget_kb, int nt) = 0;
   virtual int num_variants(int* info) = 0;
   virtual void launch_info(long* out) = 0;
   virtual double time_kernel(int which, int reps) = 0;
@@ -503,21 +508,24 @@ struct K1Variant {
   void (*fn)(const SimDev<T>*, SimState*, T*, const K1Item*, int);
 };
 
-template <typename T, int BY, int BZ, int NT>
+template <typename T, int BY, int BZ, int NT, int MINW>
 static K1Variant<T> make_variant() {
-  return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT>};
+  return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT, MINW>};
 }
 
+// stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
 template <typename T>
 static std::vector<K1Variant<T>> k1_variants();
 template <>
 std::vector<K1Variant<double>> k1_variants<double>() {
-  return {make_variant<double, 8, 32, 256>(), make_variant<double, 4, 32, 128>(), make_variant<double, 8, 16, 128>()};
+  return {make_variant<double, 8, 32, 256, 2>(), make_variant<double, 4, 32, 128, 2>(),
+          make_variant<double, 8, 16, 128, 3>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
-  return {make_variant<float, 8, 32, 256>(), make_variant<float, 16, 32, 256>(), make_variant<float, 8, 64, 256>(),
-          make_variant<float, 4, 64, 256>()};
+  return {make_variant<float, 8, 32, 256, 3>(), make_variant<float, 8, 32, 256, 2>(),
+          make_variant<float, 16, 32, 256, 2>(), make_variant<float, 8, 64, 256, 2>(),
+          make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>()};
 }
 
 constexpr int K2_NT = 256;
@@ -567,7 +575,8 @@ struct Engine : EngineBase {
     for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
       if (variants[i].lds <= (size_t)devinfo().lds) variant = i;
     require(variant >= 0, "no stencil tiling fits this device's LDS");
-    (void)hipFuncSetAttribute((const void*)&k_fft<T, FFT_NT>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+    (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+    (void)hipFuncSetAttribute((const void*)&k_fft<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
   }
   ~Engine() override {
     drop_graphs();
@@ -797,16 +806,15 @@ struct Engine : EngineBase {
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
     occ = std::max(1, occ);
     k1_grid = (unsigned)std::max(1, std::min(nk1, devinfo().cus * occ));
-    // prime new simulations: misfit + statistics, mean, misfit about the mean
-    bool any_fresh = false;
-    for (auto& s : sims) any_fresh = any_fresh || (s.used && s.fresh);
-    if (any_fresh && nk2) {
+    // prime after every rebuild (the partial-sum buffer is new): misfit +
+    // statistics, FP64 mean, misfit about the mean; X is not modified
+    if (nk2) {
       launch_k2(1);
       launch_dt(0);
       launch_k2(1);
       launch_dt(0);
-      for (auto& s : sims) s.fresh = false;
     }
+    for (auto& s : sims) s.fresh = false;
     dirty = false;
   }
   void launch_k2(int prime) {
@@ -918,6 +926,13 @@ struct Engine : EngineBase {
     require(c >= 1, "cx must be >= 1");
     variant = v;
     xchunk = c;
+    dirty = true;
+  }
+  void set_fft(int budget_kb, int nt) override {
+    require(budget_kb >= 4 && budget_kb <= 160, "FFT LDS budget must be 4..160 KiB");
+    require(nt == 128 || nt == 256, "FFT block size must be 128 or 256");
+    fft.budget = (size_t)budget_kb * 1024;
+    fft.nt = nt;
     dirty = true;
   }
   int num_variants(int* info) override {
@@ -1229,6 +1244,9 @@ int pf5d_engine_remove(pf5d_engine* e, int slot) {
 }
 int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* info) {
   return guarded([&] { *n = e->impl->num_variants(info); });
+}
+int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt) {
+  return guarded([&] { e->impl->set_fft(budget_kb, nt); });
 }
 int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx) {
   return guarded([&] { e->impl->set_tiling(variant, cx); });

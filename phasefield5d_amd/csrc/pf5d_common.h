@@ -34,6 +34,20 @@ PF5D_DEV void st4(T* p, const V4<T>& v) { *reinterpret_cast<V4<T>*>(p) = v; }
 
 PF5D_HD int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// q = idx / cols, r = idx % cols for idx < 2^20, cols < 2^11, via a float
+// reciprocal and one correction step (avoids the integer-division sequence)
+PF5D_DEV void fdivmod(int idx, int cols, float rcp, int& q, int& r) {
+  q = (int)((float)idx * rcp);
+  r = idx - q * cols;
+  if (r < 0) {
+    q -= 1;
+    r += cols;
+  } else if (r >= cols) {
+    q += 1;
+    r -= cols;
+  }
+}
+
 // non-negative IEEE values order like their bit patterns: max via integer atomics
 PF5D_DEV void atomic_max_nonneg(double* addr, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));

@@ -80,10 +80,11 @@ struct K1Smem {
   static constexpr size_t bytes = red_off + sizeof(T) * 16;
 };
 
-template <typename T, int BY, int BZ, int NT>
-__global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
-                                                T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
-                                                int nitems) {
+// MINW: minimum waves per SIMD the register allocation must allow
+template <typename T, int BY, int BZ, int NT, int MINW>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
+k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __restrict__ xmax_acc,
+          const K1Item* __restrict__ items, int nitems) {
   using S = K1Smem<T, BY, BZ>;
   extern __shared__ __align__(16) unsigned char smem[];
   V4<T>* XS = reinterpret_cast<V4<T>*>(smem + S::x_off);
@@ -116,16 +117,44 @@ __global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ si
       return MB[((slot * 5 + m) * S::MY + (j + 1)) * S::MZ + (k + 1)];
     };
 
-    auto load_plane = [&](int p) {
+    // X planes go global -> registers (issued one iteration ahead) -> LDS
+    constexpr int XPT = ((BY + 4) * (BZ + 4) + NT - 1) / NT;
+    const float rx = 1.0f / (float)xcols, rm = 1.0f / (float)(tz + 2), rt = 1.0f / (float)tz,
+                rt1 = 1.0f / (float)(tz + 1);
+    V4<T> xr[XPT];
+    auto fetch_plane = [&](int p) {
       const int gx = hx ? wrap(p, n0) : 0;
+      const int tot = xrows * xcols;
+#pragma unroll
+      for (int i = 0; i < XPT; ++i) {
+        const int idx = tid + i * NT;
+        if (idx < tot) {
+          int j, k;
+          fdivmod(idx, xcols, rx, j, k);
+          j += xj0;
+          k -= 2;
+          const int gy = hy ? wrap(item.y0 + j, n1) : 0;
+          const int gz = wrap(item.z0 + k, n2);
+          xr[i] = ld4<T>(sd.X + 4 * (((long)gx * n1 + gy) * n2 + gz));
+        }
+      }
+    };
+    auto commit_plane = [&](int p) {
       const int slot = xslot(p);
       const int tot = xrows * xcols;
-      for (int idx = tid; idx < tot; idx += NT) {
-        const int j = idx / xcols + xj0, k = idx % xcols - 2;
-        const int gy = hy ? wrap(item.y0 + j, n1) : 0;
-        const int gz = wrap(item.z0 + k, n2);
-        XAT(slot, j, k) = ld4<T>(sd.X + 4 * (((long)gx * n1 + gy) * n2 + gz));
+#pragma unroll
+      for (int i = 0; i < XPT; ++i) {
+        const int idx = tid + i * NT;
+        if (idx < tot) {
+          int j, k;
+          fdivmod(idx, xcols, rx, j, k);
+          XAT(slot, j + xj0, k - 2) = xr[i];
+        }
       }
+    };
+    auto load_plane = [&](int p) {
+      fetch_plane(p);
+      commit_plane(p);
     };
 
     // mu, Mob of plane p on the halo-1 region (corners excluded)
@@ -135,7 +164,10 @@ __global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ si
       const int mj0 = hy ? -1 : 0, mrows = hy ? ty + 2 : 1, mcols = tz + 2;
       const int tot = mrows * mcols;
       for (int idx = tid; idx < tot; idx += NT) {
-        const int j = idx / mcols + mj0, k = idx % mcols - 1;
+        int j, k;
+        fdivmod(idx, mcols, rm, j, k);
+        j += mj0;
+        k -= 1;
         const bool jh = (j < 0 || j >= ty), kh = (k < 0 || k >= tz);
         if (jh && kh) continue;
         const V4<T> xc = XAT(s0, j, k);
@@ -203,11 +235,15 @@ __global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ si
     if (hx) {
       load_plane(pbeg - 1);
       load_plane(pbeg);
+      fetch_plane(pbeg + 1);
     } else {
       load_plane(0);
     }
     for (int p = pbeg; p <= pend; ++p) {
-      if (hx) load_plane(p + 1);
+      if (hx) {
+        commit_plane(p + 1);                // fetched during the previous iteration
+        if (p + 2 <= pend + 1) fetch_plane(p + 2);  // in flight while this plane computes
+      }
       __syncthreads();
       potentials(p);
       __syncthreads();
@@ -220,7 +256,8 @@ __global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ si
         for (int i = 0; i < CPT; ++i) {
           const int c = tid + i * NT;
           if (c < ty * tz) {
-            const int j = c / tz, k = c % tz;
+            int j, k;
+            fdivmod(c, tz, rt, j, k);
             face(xslot(p - 1), ((p - 1) % 2 + 2) % 2, j, k, xslot(p), (p % 2 + 2) % 2, j, k, fx[i].v);
           }
         }
@@ -229,12 +266,14 @@ __global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ si
         const int xs = xslot(q), ms = ((q % 2) + 2) % 2;
         if (hy) {
           for (int idx = tid; idx < (ty + 1) * tz; idx += NT) {
-            const int j = idx / tz, k = idx % tz;
+            int j, k;
+            fdivmod(idx, tz, rt, j, k);
             face(xs, ms, j - 1, k, xs, ms, j, k, FY[j * BZ + k].v);
           }
         }
         for (int idx = tid; idx < ty * (tz + 1); idx += NT) {
-          const int j = idx / (tz + 1), k = idx % (tz + 1);
+          int j, k;
+          fdivmod(idx, tz + 1, rt1, j, k);
           face(xs, ms, j, k - 1, xs, ms, j, k, FZ[j * (BZ + 1) + k].v);
         }
         __syncthreads();
@@ -243,7 +282,8 @@ __global__ void __launch_bounds__(NT) k_stencil(const SimDev<T>* __restrict__ si
         for (int i = 0; i < CPT; ++i) {
           const int c = tid + i * NT;
           if (c < ty * tz) {
-            const int j = c / tz, k = c % tz;
+            int j, k;
+            fdivmod(c, tz, rt, j, k);
             const V4<T> zp = FZ[j * (BZ + 1) + k + 1], zm = FZ[j * (BZ + 1) + k];
             T acc[4];
             bool first = true;

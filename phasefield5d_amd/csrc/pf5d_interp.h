@@ -78,9 +78,37 @@ PF5D_DEV void axis_index(T u, int num, int& i, T& t) {
   }
 }
 
+// 16-byte vector of the table element type
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+};
+template <typename V>
+PF5D_DEV auto vget(const V& v, int i) {
+  if constexpr (sizeof(v.x) == 4) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+  } else {
+    return i == 0 ? v.x : v.y;
+  }
+}
+
+// The 16 corner rows are gathered in batches of one 16-byte column per corner
+// (16 independent loads in flight per batch, one wait per batch); within each
+// property the sum still runs c0000 .. c1111 left to right.
 template <typename T>
 PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
   constexpr int NP = TablePad<T>::value;
+  using VT = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  constexpr int NB = (9 + VN - 1) / VN;  // batches: 3 (FP32) or 5 (FP64)
   const int num = tv.num;
   int i0, i1, i2, i3;
   T t0, t1, t2, t3;
@@ -89,26 +117,46 @@ PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
   axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
   axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
   const T s0 = T(1) - t0, s1 = T(1) - t1, s2 = T(1) - t2, s3 = T(1) - t3;
-  const long n3 = (long)num * num * num, n2 = (long)num * num, n1 = num;
+  const int n1 = num * NP, n2 = num * num * NP, n3 = num * num * num * NP;
   const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
+  // eight row-pair pointers (b0 b1 b2); the b3 neighbour and the property column
+  // are compile-time offsets from them
+  const T* rp[8];
 #pragma unroll
-  for (int p = 0; p < 9; ++p) P[p] = T(0);
+  for (int r = 0; r < 8; ++r) rp[r] = base + ((r >> 2) & 1) * n3 + ((r >> 1) & 1) * n2 + (r & 1) * n1;
+  // weights ((f0 f1) f2) f3 as a product tree: same roundings as the reference's
+  // left-to-right product (interpolation.py:126-134), 28 multiplies
+  T w01[4], w012[8], w[16];
 #pragma unroll
-  for (int b = 0; b < 16; ++b) {
-    const int b0 = (b >> 3) & 1, b1 = (b >> 2) & 1, b2 = (b >> 1) & 1, b3 = b & 1;
-    T w = (b0 ? t0 : s0) * (b1 ? t1 : s1);
-    w = w * (b2 ? t2 : s2);
-    w = w * (b3 ? t3 : s3);
-    T g[NP];
-    load_corner<T>(base + (b0 * n3 + b1 * n2 + b2 * n1 + b3) * NP, g);
-    if (b == 0) {
+  for (int a = 0; a < 4; ++a) w01[a] = ((a >> 1) ? t0 : s0) * ((a & 1) ? t1 : s1);
 #pragma unroll
-      for (int p = 0; p < 9; ++p) P[p] = w * g[p];
-    } else {
+  for (int a = 0; a < 8; ++a) w012[a] = w01[a >> 1] * ((a & 1) ? t2 : s2);
 #pragma unroll
-      for (int p = 0; p < 9; ++p) P[p] = mad<T>(w, g[p], P[p]);
+  for (int b = 0; b < 16; ++b) w[b] = w012[b >> 1] * ((b & 1) ? t3 : s3);
+  // full 16-byte batches, then the ninth property alone (no padding fetched)
+#pragma unroll
+  for (int g = 0; g < 8 / VN; ++g) {
+    VT c[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) c[b] = *reinterpret_cast<const VT*>(rp[b >> 1] + (b & 1) * NP + g * VN);
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+      T acc = w[0] * vget(c[0], e);
+#pragma unroll
+      for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], vget(c[b], e), acc);
+      P[g * VN + e] = acc;
     }
   }
+  {
+    T c[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) c[b] = rp[b >> 1][(b & 1) * NP + 8];
+    T acc = w[0] * c[0];
+#pragma unroll
+    for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], c[b], acc);
+    P[8] = acc;
+  }
+  (void)NB;
 }
 
 template <typename T>

@@ -31,9 +31,10 @@ class Engine:
         self.slots = {}
 
     def close(self):
-        if getattr(self, "h", None) is not None and self.h.value:
-            _lib.lib().pf5d_engine_destroy(self.h)
-            self.h = None
+        h = getattr(self, "h", None)
+        self.h = None
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.pf5d_engine_destroy(h)
 
     def __del__(self):
         self.close()
@@ -90,6 +91,9 @@ class Engine:
 
     def set_tiling(self, variant, cx):
         _lib.check(_lib.lib().pf5d_engine_set_tiling(self.h, int(variant), int(cx)))
+
+    def set_fft(self, budget_kb, nt):
+        _lib.check(_lib.lib().pf5d_engine_set_fft(self.h, int(budget_kb), int(nt)))
 
     # -- readback ----------------------------------------------------------------
     def state(self, slot):

@@ -200,6 +200,28 @@ def test_fp32_engine_error(dev):
     assert drift.max() < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_retiling_midrun_keeps_trajectory(dev, precision):
+    """Changing the stencil tiling / FFT batching between steps rebuilds the work lists;
+    the simulation stays active and its trajectory is unchanged (bit-identical)."""
+    from phasefield5d_amd.engine import ST_ACTIVE, Engine
+    pb = problem(3, 16)
+    pb.X_init = developed_state(pb, amp=0.03)
+    a = Engine(precision)
+    sa = a.add_problem(pb)
+    a.step(6)
+    b = Engine(precision)
+    sb = b.add_problem(pb)
+    b.step(3)
+    b.set_tiling(len(b.variants()) - 1, 8)
+    b.set_fft(8, 128)
+    b.step(3)
+    st = b.state(sb)
+    assert st["status"] & ST_ACTIVE and st["steps"] == 6
+    assert np.array_equal(a.field(sa, "X"), b.field(sb, "X"))
+    assert np.allclose(st["mass"], pb.X_init.reshape(-1, 4).sum(axis=0), rtol=1e-12)
+
+
 def test_batched_equals_alone(dev):
     """Each simulation's result is bit-identical alone and inside a mixed batch."""
     from phasefield5d_amd.engine import Engine
