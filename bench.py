@@ -56,6 +56,51 @@ def cpu_baseline(pb, seconds=15.0, threads=None):
                       f"after 1 warm-up step, {el:.1f} s"}
 
 
+def batch_leg(steps, warmup, log_fn):
+    """64 uneven simulations (32 2D, 32 3D) in one FP32 engine, one graph per step."""
+    from phasefield5d_amd import autotune
+    from phasefield5d_amd.engine import Engine
+    from phasefield5d_amd.problem import build_problem
+    from phasefield5d_amd.workloads import batch_mix
+    import torch
+    t0 = time.perf_counter()
+    pbs = [build_problem(c, table=tab) for c, tab in batch_mix()]
+    log_fn(f"batch mix set up in {time.perf_counter() - t0:.1f} s")
+    eng = Engine("fp32")
+    for pb in pbs:
+        eng.add_problem(pb)
+    tuned = autotune.tune(eng, chunks=(16,), reps=3)
+    eng.step(warmup)
+    eng.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.sync()
+    el = time.perf_counter() - t0
+    cells = sum(pb.ncell for pb in pbs)
+    sts = eng.all_states()
+    active = sum(1 for s in sts.values() if s["status"] & 1)
+    eng.close()
+    return {"sims": len(pbs), "cells": cells, "steps": steps, "cell_updates_per_s": cells * steps / el,
+            "ms_per_step": 1e3 * el / steps, "active_after": active, "tiling": tuned}
+
+
+def serve_leg(duration, rate, max_steps, log_fn):
+    """Poisson arrivals of mixed 1D/2D/3D requests into one FP32 continuous-batching server."""
+    from phasefield5d_amd.serve import Server, audit, poisson_requests, summarize
+    reqs = poisson_requests(rate, duration, seed=0, max_steps=max_steps, audit_frac=0.05)
+    srv = Server("fp32", poll_steps=25, setup_workers=6)
+    done = srv.run(reqs, max_wall=duration * 6 + 60)
+    srv.close()
+    for r in done:
+        if r.audit:
+            audit(r)
+    out = summarize(done, srv.wall)
+    out.update({"arrival_rate_per_s": rate, "arrival_window_s": duration, "submitted": len(reqs),
+                "max_steps_per_request": max_steps, "gpu_busy_s": srv.busy})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +114,12 @@ def main():
     ap.add_argument("--calibration-copy", action="store_true",
                     help="end with a 256 MiB device-to-device copy (scales FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--fp32-only", action="store_true", help="only the FP32 fused leg (profiling runs)")
+    ap.add_argument("--no-mix", action="store_true", help="skip the 64-run batched mix leg")
+    ap.add_argument("--mix-steps", type=int, default=20)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-max-steps", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the explicit step of one simulation "
@@ -184,6 +235,14 @@ def main():
         "stage_ms": {"k1": k1_ms, "fft": fft_ms},
         "launch": info,
     }
+    if not args.fp32_only and not args.no_mix:
+        log("batched mix (64 runs)")
+        out["batched_mix"] = batch_leg(args.mix_steps, 3, log)
+        log("mix", out["batched_mix"])
+    if not args.fp32_only and not args.no_serve:
+        log("serving")
+        out["serving"] = serve_leg(args.serve_seconds, args.serve_rate, args.serve_max_steps, log)
+        log("serving", out["serving"])
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(pb, seconds=args.cpu_seconds)

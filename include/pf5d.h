@@ -123,6 +123,9 @@ int pf5d_engine_step(pf5d_engine* e, int nsteps);
 int pf5d_engine_sync(pf5d_engine* e);
 /* state[16] = dt, t, last_xmax, mass[4], steps, status, smax, smin, cmin, cmax, nonfinite, mean0, mean1 */
 int pf5d_engine_get_state(pf5d_engine* e, int slot, double* state16);
+/* the states of slots 0..min(maxn, slot count)-1 in one device->host copy (16 doubles
+ * each, layout as above; status 0 for free slots); *n = slots written */
+int pf5d_engine_get_all_states(pf5d_engine* e, int maxn, double* out, int* n);
 int pf5d_engine_set_time(pf5d_engine* e, int slot, double dt, double t);
 /* copy fields to host as float64: which = 0 X, 1 dX/dt (of the last step), 2 psi */
 int pf5d_engine_get_field(pf5d_engine* e, int slot, int which, double* host_out);

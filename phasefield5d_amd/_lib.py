@@ -54,6 +54,7 @@ _SIGS = {
     "pf5d_engine_sync": (i32, [vp]),
     "pf5d_engine_get_state": (i32, [vp, i32, P_f64]),
     "pf5d_engine_set_time": (i32, [vp, i32, f64, f64]),
+    "pf5d_engine_get_all_states": (i32, [vp, i32, P_f64, P_i32]),
     "pf5d_engine_get_field": (i32, [vp, i32, i32, P_f64]),
     "pf5d_engine_set_field": (i32, [vp, i32, P_f64]),
     "pf5d_engine_field_ptr": (vp, [vp, i32, i32]),

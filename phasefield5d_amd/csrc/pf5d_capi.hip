@@ -482,6 +482,7 @@ struct EngineBase {
   virtual void remove(int slot) = 0;
   virtual void step(int n) = 0;
   virtual void get_state(int slot, double* out) = 0;
+  virtual int get_all_states(int maxn, double* out) = 0;
   virtual void set_time(int slot, double dt, double t) = 0;
   virtual void get_field(int slot, int which, double* out) = 0;
   virtual void set_field(int slot, const double* X) = 0;
@@ -543,6 +544,7 @@ struct Engine : EngineBase {
     const pf5d_table* tab = nullptr;
     size_t bytes = 0;
     bool fresh = true;
+    double mean0[4] = {0, 0, 0, 0};  // FP64 mean composition of the field as added
   };
   std::vector<Sim> sims;
   std::vector<K1Variant<T>> variants = k1_variants<T>();
@@ -681,6 +683,16 @@ struct Engine : EngineBase {
   }
   // FP32 mode keeps X as the nearest FP32 value plus an FP32 remainder
   void upload_X(Sim& s, const double* Xh) {
+    // FP64 mean of the field (pairwise over cells is unnecessary at this size: Neumaier)
+    for (int a = 0; a < 4; ++a) {
+      double sum = 0, c = 0;
+      for (long i = 0; i < s.ncell; ++i) {
+        const double v = Xh[4 * i + a], t = sum + v;
+        c += (std::fabs(sum) >= std::fabs(v)) ? ((sum - t) + v) : ((v - t) + sum);
+        sum = t;
+      }
+      s.mean0[a] = (sum + c) / (double)s.ncell;
+    }
     std::vector<T> h(s.ncell * 4), lo(s.Xlo ? s.ncell * 4 : 0);
     for (long i = 0; i < s.ncell * 4; ++i) {
       h[i] = (T)Xh[i];
@@ -724,7 +736,8 @@ struct Engine : EngineBase {
     for (int a = 0; a < 4; ++a) {
       v.kappa[a] = (T)d.kappa[a];
       v.l[a] = d.elastic ? (T)d.l[a] : T(0);
-      v.X0[a] = (T)d.X0[a];
+      // FP32 mode: misfit about the conserved FP64 mean, so the FFT input carries no k=0 mode
+      v.X0[a] = (sizeof(T) == 4) ? (T)s.mean0[a] : (T)d.X0[a];
     }
     v.elastic = d.elastic;
     v.use_mean = (sizeof(T) == 4) ? 1 : 0;
@@ -806,11 +819,10 @@ struct Engine : EngineBase {
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
     occ = std::max(1, occ);
     k1_grid = (unsigned)std::max(1, std::min(nk1, devinfo().cus * occ));
-    // prime after every rebuild (the partial-sum buffer is new): misfit +
-    // statistics, FP64 mean, misfit about the mean; X is not modified
+    // prime after every rebuild (the partial-sum buffer is new): misfit and FP64
+    // statistics of the current X; X is not modified, so a rebuild between two
+    // steps leaves every simulation's trajectory unchanged
     if (nk2) {
-      launch_k2(1);
-      launch_dt(0);
       launch_k2(1);
       launch_dt(0);
     }
@@ -878,6 +890,23 @@ struct Engine : EngineBase {
     const double v[16] = {st.dt,  st.t,    st.last_xmax, st.mass[0], st.mass[1], st.mass[2],     st.mass[3], (double)st.steps,
                           (double)st.status, st.smax, st.smin, st.cmin, st.cmax, st.nonfinite, st.mean[0], st.mean[1]};
     std::memcpy(out, v, sizeof(v));
+  }
+  // all slots in one copy: out[16*i ...] for slot i < nslots (unused slots: status 0)
+  int get_all_states(int maxn, double* out) override {
+    if (dirty) rebuild();
+    const int n = std::min(maxn, (int)sims.size());
+    if (n <= 0) return 0;
+    std::vector<SimState> h(n);
+    HIPCHK(hipMemcpyAsync(h.data(), d_state, n * sizeof(SimState), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int i = 0; i < n; ++i) {
+      const SimState& st = h[i];
+      const double v[16] = {st.dt,          st.t,    st.last_xmax, st.mass[0], st.mass[1], st.mass[2],
+                            st.mass[3],     (double)st.steps,      (double)(sims[i].used ? st.status : 0),
+                            st.smax,        st.smin, st.cmin,      st.cmax,    st.nonfinite, st.mean[0], st.mean[1]};
+      std::memcpy(out + 16 * i, v, sizeof(v));
+    }
+    return n;
   }
   void set_time(int slot, double dt, double t) override {
     at(slot);
@@ -1259,6 +1288,9 @@ int pf5d_engine_sync(pf5d_engine* e) {
 }
 int pf5d_engine_get_state(pf5d_engine* e, int slot, double* st) {
   return guarded([&] { e->impl->get_state(slot, st); });
+}
+int pf5d_engine_get_all_states(pf5d_engine* e, int maxn, double* out, int* n) {
+  return guarded([&] { *n = e->impl->get_all_states(maxn, out); });
 }
 int pf5d_engine_set_time(pf5d_engine* e, int slot, double dt, double t) {
   return guarded([&] { e->impl->set_time(slot, dt, t); });

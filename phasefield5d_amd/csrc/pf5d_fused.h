@@ -33,9 +33,9 @@ struct SimDev {
   T inv_dx, inv_dx2;  // 1/h, 1/h^2
   T kappa[4];         // kappa_i * kappa_value
   T l[4];             // rank-1 elastic vector (zeros when elasticity is off)
-  T X0[4];            // reference composition for the misfit
+  T X0[4];            // misfit origin: X0 (FP64 mode), FP64 mean of the initial field (FP32 mode)
   int elastic;
-  int use_mean;       // misfit about the FP64 running mean (precision mode)
+  int use_mean;       // 1 when X0 holds the initial mean (the k=0 mode is removed before the FP32 FFT)
   TableView<T> tab;
   T* X;
   T* Xlo;             // FP32 mode: rounding remainder, the state is X + Xlo (null in FP64 mode)
@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
     double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
     T ref[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) ref[a] = sd.use_mean ? (T)S.mean[a] : sd.X0[a];
+    for (int a = 0; a < 4; ++a) ref[a] = sd.X0[a];
     for (long c = item.c0 + threadIdx.x; c < item.c1; c += NT) {
       V4<T> x = ld4<T>(sd.X + 4 * c);
       double xd[4];  // the state in FP64 (FP32 mode: X + Xlo)

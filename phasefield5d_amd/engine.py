@@ -105,6 +105,23 @@ class Engine:
         s["status"] = int(s["status"])
         return s
 
+    def all_states(self, maxn=4096):
+        """{slot: state dict} of every occupied slot, one device->host copy."""
+        buf = (C.c_double * (16 * maxn))()
+        n = C.c_int()
+        _lib.check(_lib.lib().pf5d_engine_get_all_states(self.h, maxn, buf, C.byref(n)))
+        out = {}
+        for i in range(n.value):
+            if i not in self.slots:
+                continue
+            v = list(buf[16 * i:16 * i + 16])
+            s = dict(zip(STATE_FIELDS, v))
+            s["mass"] = np.array([s.pop(f"mass{k}") for k in range(4)])
+            s["steps"] = int(s["steps"])
+            s["status"] = int(s["status"])
+            out[i] = s
+        return out
+
     def set_time(self, slot, dt, t):
         _lib.check(_lib.lib().pf5d_engine_set_time(self.h, slot, float(dt), float(t)))
 

@@ -78,7 +78,8 @@ def synthetic_rows(X0, T, seed, step, diffusivity):
 def table_rows(cfg, table=None, verbose=False):
     """Rows (n, 13) of the CALPHAD table for ``cfg``: given, on disk, or synthetic."""
     if table is not None:
-        return table.renamed_dataframe().to_numpy() if hasattr(table, "renamed_dataframe") else np.asarray(table)
+        # the rows array of a SyntheticTable is stable, so its dense grid is cached once
+        return table.rows if hasattr(table, "rows") else np.asarray(table)
     if not cfg.data_path and cfg.synthetic_table:
         return synthetic_rows(tuple(np.asarray(cfg.initial_composition, dtype=float)),
                               temperature_kelvin(cfg.temperature), cfg.table_seed, cfg.resolution,
@@ -98,13 +99,29 @@ def table_rows(cfg, table=None, verbose=False):
     return load_calphad_dataframe(cfg.temperature, path=path, elements=cfg.elements).to_numpy()
 
 
+_GRID_CACHE = {}
+
+
+def grid_and_tree(rows, resolution):
+    """Dense grid + KD-tree of a table, cached while the rows array is alive."""
+    key = (id(rows), float(resolution))
+    hit = _GRID_CACHE.get(key)
+    if hit is not None and hit[0] is rows:
+        return hit[1], hit[2], hit[3]
+    _, grid = build_4d_grid(rows, resolution)
+    tree, values = build_calphad_kdtree(rows)
+    if len(_GRID_CACHE) > 8:
+        _GRID_CACHE.clear()
+    _GRID_CACHE[key] = (rows, grid, tree, values)
+    return grid, tree, values
+
+
 def build_problem(cfg, table=None, verbose=False, X_init=None):
     alloy, ri, mu, nu, vi, qi = load_material_constants(cfg.atomic_radius_tag, alloy=cfg.elements)
     override = (cfg.c11, cfg.c12, cfg.c44) if cfg.c11 is not None else None
     c11, c12, c44 = load_elastic_constants(override=override)
     rows = table_rows(cfg, table, verbose)
-    _, grid = build_4d_grid(rows, cfg.resolution)
-    tree, values = build_calphad_kdtree(rows)
+    grid, tree, values = grid_and_tree(rows, cfg.resolution)
     X0 = np.asarray(cfg.initial_composition, dtype=float)
     ref = interpolate_points_host(X0[None, :], grid, cfg.resolution, tree, values)[0]
     hessian_max = float(np.abs(ref[:4]).max())

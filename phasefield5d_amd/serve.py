@@ -1,0 +1,223 @@
+"""Continuous-batching simulation server on one GPU.
+
+Requests (dimension, size, composition, kappa, T, end time) arrive over time.
+Each is set up on the host in a worker thread (table, kinetics, B(k), initial
+field), admitted into a slot of one device-resident ``Engine`` as soon as its
+HBM footprint fits, and stepped together with every other resident request:
+each step of the engine advances all resident requests by their own adaptive
+dt.  A request leaves when its device-side time passes its end time (or its
+bounds check fails); its field is copied back and its slot freed, and queued
+requests take the memory.  Membership changes rebuild the engine's work lists
+and re-capture its step graph between polls.
+
+Latency = completion time - arrival time (includes queueing and set-up).
+A sampled fraction of requests is re-run alone on an FP64 engine and its
+final composition compared with the served FP32 result ("audit").
+"""
+import math
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from .engine import ST_ACTIVE, ST_DONE, ST_HARD_OOB, Engine
+from .problem import build_problem
+
+
+@dataclass
+class Request:
+    rid: int
+    cfg: object
+    table: object
+    arrival: float                  # seconds after the server starts
+    t_end_ctld: float = 50.0        # end time in units of the request's CTLD
+    max_steps: Optional[int] = None  # optional cap: t_end <= max_steps * dt_cfl
+    audit: bool = False
+    # filled by the server
+    problem: object = None
+    t_end: float = 0.0
+    slot: int = -1
+    admitted: float = math.nan
+    finished: float = math.nan
+    steps: int = 0
+    status: int = 0
+    X: Optional[np.ndarray] = None
+    audit_err: Optional[float] = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def footprint_bytes(pb, precision):
+    """Device bytes a request holds in an engine (fields, remainder, spectrum, kernel)."""
+    w = 4 if precision == "fp32" else 8
+    n = pb.ncell
+    b = n * 4 * w * (3 if precision == "fp32" else 2)     # X, dX (+ remainder in FP32)
+    if pb.Bhat is not None:
+        nh = int(np.prod(pb.Bhat.shape))
+        b += n * 2 * w + nh * 3 * w                        # psi, s, spectrum (complex), Bhat
+    return b
+
+
+class Server:
+    def __init__(self, precision="fp32", hbm_budget=None, poll_steps=20, setup_workers=4, autotune=False):
+        self.engine = Engine(precision)
+        self.precision = precision
+        if hbm_budget is None:
+            import torch
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(free * 0.7)
+        self.hbm_budget = hbm_budget
+        self.poll_steps = poll_steps
+        self.pool = ThreadPoolExecutor(max_workers=setup_workers)
+        self.autotune = autotune
+        self.used_bytes = 0
+        self.log = []
+
+    def _setup(self, req):
+        pb = build_problem(req.cfg, table=req.table)
+        t_end = req.t_end_ctld * pb.ctld
+        if req.max_steps is not None:
+            t_end = min(t_end, req.max_steps * pb.dt_cfl)
+        req.problem, req.t_end = pb, t_end
+        return req
+
+    def run(self, requests, max_wall=None, verbose=False):
+        """Serve ``requests`` (arrival times relative to now); returns them completed."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        futures = []           # (req, future) being set up
+        ready = []             # set up, waiting for memory
+        running = {}           # slot -> req
+        done = []
+        t0 = time.perf_counter()
+        i = 0
+        busy = 0.0
+        while len(done) < len(requests):
+            now = time.perf_counter() - t0
+            if max_wall is not None and now > max_wall:
+                break
+            while i < len(pending) and pending[i].arrival <= now:
+                futures.append((pending[i], self.pool.submit(self._setup, pending[i])))
+                i += 1
+            still = []
+            for req, fut in futures:
+                if fut.done():
+                    fut.result()
+                    ready.append(req)
+                else:
+                    still.append((req, fut))
+            futures = still
+            # admission by HBM footprint, first come first served
+            admitted = False
+            while ready:
+                req = ready[0]
+                need = footprint_bytes(req.problem, self.precision)
+                if need > self.hbm_budget:
+                    raise MemoryError(f"request {req.rid} needs {need} bytes > budget {self.hbm_budget}")
+                if self.used_bytes + need > self.hbm_budget:
+                    break
+                ready.pop(0)
+                req.slot = self.engine.add_problem(req.problem, t_end=req.t_end)
+                req.admitted = time.perf_counter() - t0
+                req.extra["bytes"] = need
+                self.used_bytes += need
+                running[req.slot] = req
+                admitted = True
+            if admitted and self.autotune:
+                from . import autotune
+                autotune.tune(self.engine, chunks=(16,), reps=3)
+            if running:
+                ts = time.perf_counter()
+                self.engine.step(self.poll_steps)
+                states = self.engine.all_states()
+                busy += time.perf_counter() - ts
+                now = time.perf_counter() - t0
+                for slot, st in states.items():
+                    req = running.get(slot)
+                    if req is None or st["status"] & ST_ACTIVE:
+                        continue
+                    req.steps, req.status = st["steps"], st["status"]
+                    req.X = self.engine.field(slot, "X")
+                    req.finished = now
+                    self.engine.remove(slot)
+                    self.used_bytes -= req.extra["bytes"]
+                    del running[slot]
+                    done.append(req)
+                    if verbose:
+                        print(f"  request {req.rid} {req.problem.shape} done: {req.steps} steps, "
+                              f"latency {req.latency:.2f} s, status {req.status}", flush=True)
+            else:
+                nxt = pending[i].arrival if i < len(pending) else None
+                if nxt is not None and not futures and not ready:
+                    time.sleep(max(0.0, min(0.01, nxt - (time.perf_counter() - t0))))
+                else:
+                    time.sleep(0.001)
+        self.wall = time.perf_counter() - t0
+        self.busy = busy
+        return done
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+        self.engine.close()
+
+
+def audit(req, precision="fp64"):
+    """Re-run a served request alone on an FP64 engine; max |X_served - X_fp64|."""
+    eng = Engine(precision)
+    slot = eng.add_problem(req.problem, t_end=req.t_end)
+    while True:
+        eng.step(200)
+        st = eng.state(slot)
+        if not st["status"] & ST_ACTIVE:
+            break
+    X = eng.field(slot, "X")
+    eng.close()
+    req.audit_err = float(np.abs(X - req.X).max())
+    req.extra["audit_steps"] = st["steps"]
+    return req.audit_err
+
+
+def poisson_requests(rate, duration, seed=0, max_steps=500, audit_frac=0.05, t_end_ctld=50.0):
+    """Arrivals at Poisson ``rate`` for ``duration`` s.  Dimension 1/2/3 with probability
+    0.3/0.5/0.2: 1D mw 20-200 x ppw 8-16; 2D sides 128-512; 3D sides 64-256 (sides drawn
+    as multiples of 16); composition and kappa as in the batched mix; end time 50 CTLD,
+    capped at ``max_steps`` CFL-limited steps when given."""
+    from .workloads import _mix_config, draw_spinodal_composition, table_for
+    rng = np.random.default_rng(seed)
+    t, reqs, rid = 0.0, [], 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        u = rng.uniform()
+        T = int(rng.choice([873, 1000]))
+        X0 = draw_spinodal_composition(rng, T)
+        kappa = float(10 ** rng.uniform(-16, -14))
+        if u < 0.3:
+            mw, ppw = int(rng.integers(20, 201)), int(rng.integers(8, 17))
+            cfg = _mix_config(1, mw * ppw, X0, kappa, T, rid, ppw=ppw)
+        elif u < 0.8:
+            cfg = _mix_config(2, 16 * int(rng.integers(8, 33)), X0, kappa, T, rid)
+        else:
+            cfg = _mix_config(3, 16 * int(rng.integers(4, 17)), X0, kappa, T, rid)
+        reqs.append(Request(rid, cfg, table_for(T), t, t_end_ctld=t_end_ctld, max_steps=max_steps,
+                            audit=bool(rng.uniform() < audit_frac)))
+        rid += 1
+    return reqs
+
+
+def summarize(done, wall):
+    lat = np.array([r.latency for r in done])
+    cells = sum(r.problem.ncell * r.steps for r in done)
+    errs = [r.audit_err for r in done if r.audit_err is not None]
+    return {"requests": len(done), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "wall_s": wall, "cell_updates": int(cells),
+            "cell_updates_per_s": cells / wall, "audited": len(errs),
+            "audit_max_err": max(errs) if errs else None,
+            "failed": sum(1 for r in done if r.status & ST_HARD_OOB),
+            "completed": sum(1 for r in done if r.status & ST_DONE)}

@@ -36,6 +36,66 @@ def workload_config(name, seed=0, table_seed=0, extra=()):
     return cfg
 
 
+TABLE_X0 = "0.1,0.2,0.2,0.3"   # composition the per-temperature tables are drawn spinodal at
+TABLE_SEEDS = {873: 873, 1000: 1000}
+
+
+def _mix_config(dim, side, X0, kappa, T, seed, ppw=16):
+    mw = max(1, side // ppw)
+    args = COMMON + ["--system_dim", str(dim), "--mw", str(mw), "--ppw", str(side // mw),
+                     "--direction", "0,0,1" if dim == 2 else "1,0,0",
+                     "--initial_composition", ",".join(f"{v:.6f}" for v in X0), "--kappa_value", f"{kappa:.6e}",
+                     "--temperature", f"{T}K", "--seed", str(seed), "--table_seed", str(TABLE_SEEDS[T])]
+    return parse_args_to_config(args)
+
+
+def draw_spinodal_composition(rng, T, max_sum=0.85, tries=200):
+    """X0 ~ Dirichlet(2,2,2,2,2) restricted to sum(X) <= max_sum and to the spinodal
+    region of the T table (compute_initial_kinetics must find a growing mode)."""
+    from .problem import build_problem
+    for _ in range(tries):
+        x5 = rng.dirichlet([2, 2, 2, 2, 2])
+        X0 = x5[1:]
+        if X0.sum() > max_sum:
+            continue
+        cfg = _mix_config(1, 32, X0, 1e-15, T, 0)
+        tab = table_for(T)
+        try:
+            build_problem(cfg, table=tab)
+        except ValueError:
+            continue
+        return X0
+    raise RuntimeError("no spinodal composition drawn")
+
+
+_TABLES = {}
+
+
+def table_for(T):
+    """The seeded synthetic table of temperature T (spinodal at TABLE_X0), rows as the loader returns them."""
+    if T not in _TABLES:
+        from .thermodynamics.synthetic import make_synthetic_table
+        X0 = np.array([float(v) for v in TABLE_X0.split(",")])
+        _TABLES[T] = make_synthetic_table(X0, T=float(T), seed=TABLE_SEEDS[T], step=0.025)
+    return _TABLES[T]
+
+
+def batch_mix(n2d=32, n3d=32, seed=0, sides2d=(128, 192, 256, 384, 512), sides3d=(64, 96, 128)):
+    """The 64-run batch: 2D sides from sides2d, 3D sides from sides3d (uniform draws),
+    X0 from a spinodal-restricted Dirichlet(2,...,2), kappa log-uniform in [1e-16, 1e-14],
+    T in {873 K, 1000 K} with one seeded table each.  Returns (config, table) pairs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n2d + n3d):
+        dim = 2 if i < n2d else 3
+        side = int(rng.choice(sides2d if dim == 2 else sides3d))
+        T = int(rng.choice([873, 1000]))
+        X0 = draw_spinodal_composition(rng, T)
+        kappa = float(10 ** rng.uniform(-16, -14))
+        out.append((_mix_config(dim, side, X0, kappa, T, seed * 1000 + i), table_for(T)))
+    return out
+
+
 def small_config(dim, n, X0="0.1,0.2,0.2,0.3", kappa=5e-16, seed=0, elastic=True, ppw=None):
     """A reduced-size run of the same physics (parity tests): n cells per axis."""
     ppw = ppw or (8 if n % 8 == 0 else n)
