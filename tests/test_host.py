@@ -144,6 +144,22 @@ def test_problem_setup_workloads():
     assert pb.X_init.shape == (160, 4) and abs(pb.X_init.mean(axis=0) - pb.X0).max() < 1e-3
 
 
+def test_save_schedule_spacing():
+    """solver/io.py:42-109: first and last steps save; early frames every ctld/early_frames."""
+    from phasefield5d_amd.simulate import SaveSchedule
+    s = SaveSchedule(ctld=10.0, total_steps=1000, early_frames=10, burst_duration_ctld=2, burst_frames=4,
+                     tail_dt_ctld=1.0, snapshot_factor=2, every_data=None, every_snap=None)
+    assert s(0, 0.0) == (True, True)
+    saves = [k for k in range(1, 999) if s(k, k * 0.1)[0]]
+    # early stage (t < 10): the first step after the initial save, then one save per 1.0 of time
+    early = [k for k in saves if k * 0.1 < 10.0]
+    assert early == [1, 10, 20, 30, 40, 50, 60, 70, 80, 90]
+    # burst stage (10 <= t < 30): 4 frames over 20 -> every 5.0
+    burst = [k for k in saves if 10.0 <= k * 0.1 < 30.0]
+    assert burst == [100, 150, 200, 250]
+    assert s(999, 99.9) == (True, True)
+
+
 def test_tsv_roundtrip(tmp_path):
     from phasefield5d_amd.thermodynamics.io import load_calphad_dataframe
     from phasefield5d_amd.thermodynamics.synthetic import make_synthetic_table, write_synthetic_table
