@@ -32,6 +32,50 @@ PF5D_DEV V4<T> ld4(const T* p) { return *reinterpret_cast<const V4<T>*>(p); }
 template <typename T>
 PF5D_DEV void st4(T* p, const V4<T>& v) { *reinterpret_cast<V4<T>*>(p) = v; }
 
+// Explicit address spaces: pointers that reach a kernel through a descriptor
+// array are generic, and generic accesses compile to FLAT instructions, which
+// wait on both the vector-memory and the LDS counters.  These helpers state
+// the space so global data uses global_load/store and LDS data ds_read/write.
+#define PF5D_GLOBAL __attribute__((address_space(1)))
+#define PF5D_LDS __attribute__((address_space(3)))
+// (class types cannot be copied out of a qualified address space, so the
+// bytes move as a vector of 32-bit words; the copies fold away)
+template <int NB>
+struct Words {
+  using type = unsigned int __attribute__((ext_vector_type(NB / 4)));
+};
+template <>
+struct Words<4> {
+  using type = unsigned int;
+};
+template <typename V>
+PF5D_DEV V gld(const void* p) {
+  using W = typename Words<sizeof(V)>::type;
+  const W w = *(const PF5D_GLOBAL W*)p;
+  V v;
+  __builtin_memcpy(&v, &w, sizeof(V));
+  return v;
+}
+template <typename V>
+PF5D_DEV void gst(void* p, const V& v) {
+  using W = typename Words<sizeof(V)>::type;
+  W w;
+  __builtin_memcpy(&w, &v, sizeof(V));
+  *(PF5D_GLOBAL W*)p = w;
+}
+template <typename V>
+PF5D_DEV V sld(const void* p) {
+  using W = typename Words<sizeof(V)>::type;
+  const W w = *(const PF5D_LDS W*)p;
+  V v;
+  __builtin_memcpy(&v, &w, sizeof(V));
+  return v;
+}
+template <typename T>
+PF5D_DEV V4<T> ld4g(const T* p) { return gld<V4<T>>(p); }
+template <typename T>
+PF5D_DEV void st4g(T* p, const V4<T>& v) { gst<V4<T>>(p, v); }
+
 PF5D_HD int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // q = idx / cols, r = idx % cols for idx < 2^20, cols < 2^11, via a float

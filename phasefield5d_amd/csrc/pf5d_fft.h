@@ -213,12 +213,12 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
       }
       cx<T> v;
       if (I.kind == FK_R2C || I.kind == FK_LINE) {
-        v = {reinterpret_cast<const T*>(I.in)[g * I.ls_in + (long)m * I.ps_in], T(0)};
+        v = {gld<T>(reinterpret_cast<const T*>(I.in) + (g * I.ls_in + (long)m * I.ps_in)), T(0)};
       } else if (I.kind == FK_C2R) {
         const cx<T>* C = reinterpret_cast<const cx<T>*>(I.in) + g * I.ls_in;
-        v = (m < nh) ? conjg(C[(long)m * I.ps_in]) : C[(long)(n - m) * I.ps_in];
+        v = (m < nh) ? conjg(gld<cx<T>>(C + (long)m * I.ps_in)) : gld<cx<T>>(C + (long)(n - m) * I.ps_in);
       } else {
-        v = reinterpret_cast<const cx<T>*>(I.in)[g * I.ls_in + (long)m * I.ps_in];
+        v = gld<cx<T>>(reinterpret_cast<const cx<T>*>(I.in) + (g * I.ls_in + (long)m * I.ps_in));
         if (I.kind == FK_INV) v = conjg(v);
       }
       buf0[g * n + m] = v;
@@ -231,9 +231,9 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
         const int g = idx / n, m = idx - g * n;
         T b;
         if (I.kind == FK_LINE)
-          b = reinterpret_cast<const T*>(I.bhat)[(m < nh) ? m : n - m];
+          b = gld<T>(reinterpret_cast<const T*>(I.bhat) + ((m < nh) ? m : n - m));
         else
-          b = reinterpret_cast<const T*>(I.bhat)[g * I.ls_b + (long)m * I.ps_b];
+          b = gld<T>(reinterpret_cast<const T*>(I.bhat) + (g * I.ls_b + (long)m * I.ps_b));
         const cx<T> z = res[idx];
         res[idx] = {z.x * b, -z.y * b};
       }
@@ -256,11 +256,11 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
       }
       const cx<T> v = res[g * n + m];
       if (I.kind == FK_C2R || I.kind == FK_LINE) {
-        reinterpret_cast<T*>(I.out)[g * I.ls_out + (long)m * I.ps_out] = v.x;
+        gst<T>(reinterpret_cast<T*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), v.x);
       } else {
         cx<T> o = v;
         if (I.kind == FK_INV || I.kind == FK_CONV) o = conjg(o);
-        reinterpret_cast<cx<T>*>(I.out)[g * I.ls_out + (long)m * I.ps_out] = o;
+        gst<cx<T>>(reinterpret_cast<cx<T>*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), o);
       }
     }
     __syncthreads();

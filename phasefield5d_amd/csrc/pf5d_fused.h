@@ -77,7 +77,13 @@ struct K1Smem {
   static constexpr size_t fz_off = fy_off + fy_sz;
   static constexpr size_t fz_sz = sizeof(V4<T>) * BY * (BZ + 1);
   static constexpr size_t red_off = fz_off + fz_sz;
-  static constexpr size_t bytes = red_off + sizeof(T) * 16;
+  static constexpr size_t red_sz = sizeof(T) * 16;
+  // table nodes of the plane's interpolation box (rows of NPAD), staged once per plane
+  static constexpr int BOXMAX = sizeof(T) == 4 ? 256 : 128;
+  static constexpr size_t box_off = (red_off + red_sz + 15) / 16 * 16;
+  static constexpr size_t box_sz = sizeof(T) * TablePad<T>::value * BOXMAX;
+  static constexpr size_t bred_off = box_off + box_sz;      // int [16][8] wave partials + [8] result
+  static constexpr size_t bytes = bred_off + sizeof(int) * (16 * 8 + 8);
 };
 
 // MINW: minimum waves per SIMD the register allocation must allow
@@ -93,7 +99,12 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
   V4<T>* FY = reinterpret_cast<V4<T>*>(smem + S::fy_off);
   V4<T>* FZ = reinterpret_cast<V4<T>*>(smem + S::fz_off);
   T* RED = reinterpret_cast<T*>(smem + S::red_off);
+  T* BOX = reinterpret_cast<T*>(smem + S::box_off);
+  int* BRED = reinterpret_cast<int*>(smem + S::bred_off);
   constexpr int CPT = (BY * BZ + NT - 1) / NT;
+  constexpr int NP = TablePad<T>::value;
+  using VT = typename Vec16<T>::type;
+  constexpr int NCOL = NP / Vec16<T>::n;  // 16-byte columns per table row
   const int tid = threadIdx.x;
 
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -135,7 +146,7 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
           k -= 2;
           const int gy = hy ? wrap(item.y0 + j, n1) : 0;
           const int gz = wrap(item.z0 + k, n2);
-          xr[i] = ld4<T>(sd.X + 4 * (((long)gx * n1 + gy) * n2 + gz));
+          xr[i] = ld4g<T>(sd.X + 4 * (((long)gx * n1 + gy) * n2 + gz));
         }
       }
     };
@@ -157,12 +168,35 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
       commit_plane(p);
     };
 
+    // Table nodes used by the plane's linear-region points: the bounding box of
+    // their lower cube corners is reduced across the block (wave shuffles + LDS);
+    // when it holds at most BOXMAX nodes they are copied to LDS once and every
+    // point of the plane interpolates from there (same arithmetic, bit-identical),
+    // otherwise the points gather from the table in HBM.
+    auto stage_box = [&](int s0, int mj0, int mcols, int tot) -> NodeBox {
+      int v[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) v[a] = 0x7fffffff;
+      for (int idx = tid; idx < tot; idx += NT) {
+        int j, k;
+        fdivmod(idx, mcols, rm, j, k);
+        j += mj0;
+        k -= 1;
+        if ((j < 0 || j >= ty) && (k < 0 || k >= tz)) continue;
+        const V4<T> xc = XAT(s0, j, k);
+        if (!(((xc.v[0] + xc.v[1]) + xc.v[2]) + xc.v[3] <= sd.tab.thresh)) continue;
+        node_box_point<T>(sd.tab, xc.v, v);
+      }
+      return stage_node_box<T, NT, S::BOXMAX>(sd.tab, v, BOX, BRED, tid);
+    };
+
     // mu, Mob of plane p on the halo-1 region (corners excluded)
     auto potentials = [&](int p) {
       const int s0 = xslot(p), sp = xslot(p + 1), sm = xslot(p - 1), ms = ((p % 2) + 2) % 2;
       const int gx = hx ? wrap(p, n0) : 0;
       const int mj0 = hy ? -1 : 0, mrows = hy ? ty + 2 : 1, mcols = tz + 2;
       const int tot = mrows * mcols;
+      const NodeBox nb = stage_box(s0, mj0, mcols, tot);
       for (int idx = tid; idx < tot; idx += NT) {
         int j, k;
         fdivmod(idx, mcols, rm, j, k);
@@ -174,7 +208,7 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
         T xa[4], P[9];
 #pragma unroll
         for (int a = 0; a < 4; ++a) xa[a] = xc.v[a];
-        interp9<T>(sd.tab, xa, P);
+        interp9_staged<T>(sd.tab, BOX, nb, xa, P);
         T acc[4];
         bool first = true;
         if (hx) {
@@ -198,7 +232,7 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
         if (sd.elastic) {
           const int gy = hy ? wrap(item.y0 + j, n1) : 0;
           const int gz = wrap(item.z0 + k, n2);
-          psi = sd.psi[((long)gx * n1 + gy) * n2 + gz];
+          psi = gld<T>(sd.psi + (((long)gx * n1 + gy) * n2 + gz));
         }
         V4<T> mu;
 #pragma unroll
@@ -307,7 +341,7 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
               tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
             }
             const int gy = hy ? item.y0 + j : 0;
-            st4<T>(sd.dX + 4 * (((long)gx * n1 + gy) * n2 + item.z0 + k), o);
+            st4g<T>(sd.dX + 4 * (((long)gx * n1 + gy) * n2 + item.z0 + k), o);
           }
         }
       }
@@ -474,10 +508,10 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
 #pragma unroll
     for (int a = 0; a < 4; ++a) ref[a] = sd.X0[a];
     for (long c = item.c0 + threadIdx.x; c < item.c1; c += NT) {
-      V4<T> x = ld4<T>(sd.X + 4 * c);
+      V4<T> x = ld4g<T>(sd.X + 4 * c);
       double xd[4];  // the state in FP64 (FP32 mode: X + Xlo)
       if (sd.Xlo) {
-        const V4<T> lo = ld4<T>(sd.Xlo + 4 * c);
+        const V4<T> lo = ld4g<T>(sd.Xlo + 4 * c);
 #pragma unroll
         for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a] + (double)lo.v[a];
       } else {
@@ -485,7 +519,7 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
         for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
       }
       if (run && !prime) {
-        const V4<T> d = ld4<T>(sd.dX + 4 * c);
+        const V4<T> d = ld4g<T>(sd.dX + 4 * c);
         if (!sd.Xlo) {
           // FP64: composition_change *= dt; X += composition_change; clamp (simulate.py:316-320)
 #pragma unroll
@@ -498,7 +532,7 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
           }
 #pragma unroll
           for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
-          st4<T>(sd.X + 4 * c, x);
+          st4g<T>(sd.X + 4 * c, x);
         } else {
           // FP32 state with an FP32 remainder: update and clamp in FP64, then split
           // into the nearest FP32 value and what it leaves out, so increments below
@@ -517,15 +551,15 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
             x.v[a] = (T)xd[a];
             lo.v[a] = (T)(xd[a] - (double)x.v[a]);
           }
-          st4<T>(sd.X + 4 * c, x);
-          st4<T>(sd.Xlo + 4 * c, lo);
+          st4g<T>(sd.X + 4 * c, x);
+          st4g<T>(sd.Xlo + 4 * c, lo);
         }
       }
       if (sd.elastic && run) {
         T acc = (T)(xd[0] - (double)ref[0]) * sd.l[0];
 #pragma unroll
         for (int a = 1; a < 4; ++a) acc = mad<T>((T)(xd[a] - (double)ref[a]), sd.l[a], acc);
-        sd.s[c] = acc;
+        gst<T>(sd.s + c, acc);
       }
       double csum = 0;
       bool fin = true;

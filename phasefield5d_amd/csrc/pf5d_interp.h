@@ -102,23 +102,17 @@ PF5D_DEV auto vget(const V& v, int i) {
 
 // The 16 corner rows are gathered in batches of one 16-byte column per corner
 // (16 independent loads in flight per batch, one wait per batch); within each
-// property the sum still runs c0000 .. c1111 left to right.
-template <typename T>
-PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
+// property the sum still runs c0000 .. c1111 left to right.  ``base`` points at
+// the row of the lower corner; n1, n2, n3 are the element strides of axes 2, 1, 0
+// (the stride of axis 3 is one row).  The rows may live in the HBM table or in
+// an LDS copy of a box of nodes: the arithmetic is the same.
+template <typename T, bool FROM_LDS = false>
+PF5D_DEV void interp_core(const T* base, int n1, int n2, int n3, T t0, T t1, T t2, T t3, T P[9]) {
   constexpr int NP = TablePad<T>::value;
   using VT = typename Vec16<T>::type;
   constexpr int VN = Vec16<T>::n;
   constexpr int NB = (9 + VN - 1) / VN;  // batches: 3 (FP32) or 5 (FP64)
-  const int num = tv.num;
-  int i0, i1, i2, i3;
-  T t0, t1, t2, t3;
-  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
-  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
-  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
-  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
   const T s0 = T(1) - t0, s1 = T(1) - t1, s2 = T(1) - t2, s3 = T(1) - t3;
-  const int n1 = num * NP, n2 = num * num * NP, n3 = num * num * num * NP;
-  const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
   // eight row-pair pointers (b0 b1 b2); the b3 neighbour and the property column
   // are compile-time offsets from them
   const T* rp[8];
@@ -138,7 +132,13 @@ PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
   for (int g = 0; g < 8 / VN; ++g) {
     VT c[16];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) c[b] = *reinterpret_cast<const VT*>(rp[b >> 1] + (b & 1) * NP + g * VN);
+    for (int b = 0; b < 16; ++b) {
+      const T* q = rp[b >> 1] + (b & 1) * NP + g * VN;
+      if constexpr (FROM_LDS)
+        c[b] = sld<VT>(q);
+      else
+        c[b] = gld<VT>(q);
+    }
 #pragma unroll
     for (int e = 0; e < VN; ++e) {
       T acc = w[0] * vget(c[0], e);
@@ -146,17 +146,165 @@ PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
       for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], vget(c[b], e), acc);
       P[g * VN + e] = acc;
     }
+    // keep the scheduler from hoisting the next batch's loads above this one's
+    // sums: at most 16 rows are held in registers at a time
+    __builtin_amdgcn_sched_barrier(0);
   }
   {
     T c[16];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) c[b] = rp[b >> 1][(b & 1) * NP + 8];
+    for (int b = 0; b < 16; ++b) {
+      const T* q = rp[b >> 1] + (b & 1) * NP + 8;
+      if constexpr (FROM_LDS)
+        c[b] = sld<T>(q);
+      else
+        c[b] = gld<T>(q);
+    }
     T acc = w[0] * c[0];
 #pragma unroll
     for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], c[b], acc);
     P[8] = acc;
   }
   (void)NB;
+}
+
+template <typename T>
+PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  const int num = tv.num;
+  int i0, i1, i2, i3;
+  T t0, t1, t2, t3;
+  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
+  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
+  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
+  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
+  const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
+  interp_core<T>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);
+}
+
+// A box of table nodes [lo_d, lo_d + ext_d) staged in LDS, rows of NPAD elements
+// in the table's own row-major node order.
+struct NodeBox {
+  int lo[4];
+  int st[4];   // node strides: st[3] = 1, st[2] = ext3, st[1] = ext3*ext2, st[0] = ext3*ext2*ext1
+  int vol;     // number of nodes (0: not staged)
+};
+
+template <typename T>
+PF5D_DEV void interp_linear_box(const TableView<T>& tv, const T* box, const NodeBox& nb, const T X[4], T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  const int num = tv.num;
+  int i0, i1, i2, i3;
+  T t0, t1, t2, t3;
+  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
+  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
+  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
+  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
+  const int q = (i0 - nb.lo[0]) * nb.st[0] + (i1 - nb.lo[1]) * nb.st[1] + (i2 - nb.lo[2]) * nb.st[2] + (i3 - nb.lo[3]);
+  interp_core<T, true>(box + q * NP, nb.st[2] * NP, nb.st[1] * NP, nb.st[0] * NP, t0, t1, t2, t3, P);
+}
+
+// interp9 with the linear region read from an LDS node box when one is staged
+// (nb.vol > 0), else gathered from the table in HBM; the same arithmetic either
+// way, with LDS (ds_read) or global loads chosen at compile time
+template <typename T>
+PF5D_DEV void interp9_staged(const TableView<T>& tv, const T* box, const NodeBox& nb, const T X[4], T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  const T s = ((X[0] + X[1]) + X[2]) + X[3];
+  if (!(s <= tv.thresh)) {
+    interp_nearest<T>(tv, X, P);
+    return;
+  }
+  const int num = tv.num;
+  int i0, i1, i2, i3;
+  T t0, t1, t2, t3;
+  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
+  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
+  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
+  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
+  if (nb.vol > 0) {
+    const T* base = box + ((i0 - nb.lo[0]) * nb.st[0] + (i1 - nb.lo[1]) * nb.st[1] + (i2 - nb.lo[2]) * nb.st[2] +
+                           (i3 - nb.lo[3])) * NP;
+    interp_core<T, true>(base, nb.st[2] * NP, nb.st[1] * NP, nb.st[0] * NP, t0, t1, t2, t3, P);
+  } else {
+    const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
+    interp_core<T, false>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);
+  }
+}
+
+// floor index of the lower cube corner on one axis (same rule as axis_index)
+template <typename T>
+PF5D_DEV int floor_index(T u, int num) {
+  return (u <= T(0)) ? 0 : ((u >= T(num - 1)) ? num - 2 : (int)u);
+}
+
+// fold one linear-region point into a thread's box bounds: v[a] = min lower
+// corner index on axis a, v[4 + a] = min of its negation (start from INT_MAX)
+template <typename T>
+PF5D_DEV void node_box_point(const TableView<T>& tab, const T X[4], int v[8]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int f = floor_index<T>(X[a] * tab.inv_step, tab.num);
+    v[a] = min(v[a], f);
+    v[4 + a] = min(v[4 + a], -f);
+  }
+}
+
+// Block-wide node box: the bounds in v are reduced over the block (wave
+// shuffles, then one partial per wave in bred, an int[(NT/64)*8 + 8] LDS
+// array); the box spans [min, max + 1] on every axis.  With at most BOXMAX
+// nodes its rows are copied from the table to box (LDS, rows of NPAD) and the
+// box is returned, otherwise vol = 0 (points then gather from the table).
+// Every thread of the block calls it: it synchronises the block.
+template <typename T, int NT, int BOXMAX>
+PF5D_DEV NodeBox stage_node_box(const TableView<T>& tab, int v[8], T* box, int* bred, int tid) {
+  constexpr int NP = TablePad<T>::value;
+  using VT = typename Vec16<T>::type;
+  constexpr int NCOL = NP / Vec16<T>::n;  // 16-byte columns per row
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int a = 0; a < 8; ++a) v[a] = min(v[a], __shfl_xor(v[a], off, 64));
+  if ((tid & 63) == 0)
+#pragma unroll
+    for (int a = 0; a < 8; ++a) bred[(tid >> 6) * 8 + a] = v[a];
+  __syncthreads();
+  if (tid < 8) {
+    int m = bred[tid];
+    for (int w = 1; w < NW; ++w) m = min(m, bred[w * 8 + tid]);
+    bred[NW * 8 + tid] = m;
+  }
+  __syncthreads();
+  NodeBox nb;
+  nb.vol = 1;
+  int ext[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int lo = bred[NW * 8 + a], hi = -bred[NW * 8 + 4 + a];
+    nb.lo[a] = lo;
+    ext[a] = (lo > hi) ? 0 : hi - lo + 2;
+    nb.vol *= ext[a];
+  }
+  if (nb.vol == 0 || nb.vol > BOXMAX) {
+    nb.vol = 0;
+    return nb;
+  }
+  nb.st[3] = 1;
+  nb.st[2] = ext[3];
+  nb.st[1] = ext[3] * ext[2];
+  nb.st[0] = nb.st[1] * ext[1];
+  const int num = tab.num;
+  for (int e = tid; e < nb.vol * NCOL; e += NT) {
+    const int node = e / NCOL, col = e - node * NCOL;
+    const int q0 = node / nb.st[0], r0 = node - q0 * nb.st[0];
+    const int q1 = r0 / nb.st[1], r1 = r0 - q1 * nb.st[1];
+    const int q2 = r1 / nb.st[2], q3 = r1 - q2 * nb.st[2];
+    const long row = ((((long)(nb.lo[0] + q0) * num + nb.lo[1] + q1) * num + nb.lo[2] + q2) * num + nb.lo[3] + q3);
+    reinterpret_cast<VT*>(box)[e] = gld<VT>(tab.data + row * NP + col * Vec16<T>::n);
+  }
+  __syncthreads();
+  return nb;
 }
 
 template <typename T>
@@ -203,7 +351,8 @@ PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
     return;
   }
   T g[NP];
-  load_corner<T>(tv.data + bidx * NP, g);
+#pragma unroll
+  for (int p = 0; p < 9; ++p) g[p] = gld<T>(tv.data + bidx * NP + p);
 #pragma unroll
   for (int p = 0; p < 9; ++p) P[p] = g[p];
 }
