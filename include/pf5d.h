@@ -139,8 +139,9 @@ int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes);
 int pf5d_engine_launch_info(pf5d_engine* e, long out5[5]);
 
 /* average device time of one launch of a stage (HIP events on the engine stream):
- * which = 0 fused stencil K1, 1 the FFT passes, 2 dt controller, 3 update kernel in
- * its statistics-only mode.  The stencil is idempotent, so timing it leaves the
+ * which = 0 stencil stage K1 (either form), 1 the FFT passes, 2 dt controller, 3 update
+ * kernel in its statistics-only mode, 4 / 5 the potentials / flux pass of the two-pass
+ * stencil form (when selected).  The stencil is idempotent, so timing it leaves the
  * state of the step unchanged. */
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms);
 

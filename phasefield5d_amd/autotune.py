@@ -1,8 +1,9 @@
-"""Autotuner for the fused stencil tiling (and, through it, the FFT line batching).
+"""Autotuner for the stencil stage (and, through it, the FFT line batching).
 
-For an engine holding its simulations, every compiled K1 tiling variant
-(BY x BZ tile, threads, register budget) is combined with x-chunk lengths for
-3D marching and timed with HIP events on the engine stream
+For an engine holding its simulations, every compiled K1 variant -- the
+single-pass fused tilings (BY x BZ tile, threads, register budget) combined
+with x-chunk lengths for 3D marching, and the two-pass form (reported as
+tile 0 x 64) -- is timed with HIP events on the engine stream
 (``Engine.time_kernel``); the fastest is kept.  Results are cached per
 (precision, multiset of simulation shapes) in a JSON file next to the library
 so later engines with the same mix start tuned.
@@ -54,7 +55,8 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
     has3d = any(len(pb.shape) == 3 for pb, _ in engine.slots.values())
     results = []
     for v, (by, bz, nt) in enumerate(engine.variants()):
-        for cx in (chunks if has3d else chunks[:1]):
+        # by == 0 is the two-pass form (no x marching, so no chunk length)
+        for cx in (chunks if has3d and by else chunks[:1]):
             try:
                 engine.set_tiling(v, cx)
                 ms = engine.time_kernel("k1", reps=reps)

@@ -18,6 +18,7 @@
 #include "pf5d_flux.h"
 #include "pf5d_fused.h"
 #include "pf5d_interp.h"
+#include "pf5d_split.h"
 
 using namespace pf5d;
 
@@ -197,6 +198,12 @@ struct FftPlans {
     if (it != index.end()) return it->second;
     std::vector<int> rad = factorize(n);
     require((int)rad.size() <= FFT_MAXPASS, "FFT length has too many factors: " + std::to_string(n));
+    if (ct_length(n)) {
+      // the kernel's compile-time plan for this length must be this factorisation
+      bool same = ct_npass(n) == (int)rad.size();
+      for (int p = 0; same && p < (int)rad.size(); ++p) same = ct_radix(n, p) == rad[p];
+      require(same, "compile-time FFT plan differs from the host plan for n = " + std::to_string(n));
+    }
     LinePlanDev pl{};
     pl.n = n;
     pl.npass = (int)rad.size();
@@ -261,18 +268,23 @@ struct FftBatch {
   size_t budget = 16 * 1024;
   int nt = 128;
 
-  int pick_g(int n, long nlines) const {
-    long g = (long)(budget / (2 * (size_t)n * sizeof(cx<T>)));
+  static bool paired(int kind) { return kind == FK_R2C || kind == FK_C2R; }
+  // lines per item: two line buffers within the LDS budget (real-line stages
+  // carry two lines per complex line)
+  int pick_g(int n, long nlines, int kind) const {
+    long g = (long)(budget / (2 * (size_t)line_stride(n) * sizeof(cx<T>)));
     g = std::max<long>(1, std::min<long>(g, 64));
+    if (paired(kind)) g *= 2;
     return (int)std::min<long>(g, nlines);
   }
   void push(int st, FftItem it, int n) {
     stage[st].push_back(it);
-    lds[st] = std::max(lds[st], 2 * (size_t)it.G * n * sizeof(cx<T>));
+    const size_t gl = paired(it.kind) ? (it.G + 1) / 2 : it.G;
+    lds[st] = std::max(lds[st], ((size_t)n + 2 * gl * line_stride(n)) * sizeof(cx<T>));
   }
   // lines [0, nlines) with line l at base + l*ls; split into groups of G
   void add_lines(int st, FftItem proto, int n, long nlines, long line_unit_in, long line_unit_out, long line_unit_b) {
-    const int G = pick_g(n, nlines);
+    const int G = pick_g(n, nlines, proto.kind);
     for (long l0 = 0; l0 < nlines; l0 += G) {
       FftItem it = proto;
       it.G = (int)std::min<long>(G, nlines - l0);
@@ -504,14 +516,21 @@ struct pf5d_engine {
 
 template <typename T>
 struct K1Variant {
-  int by, bz, nt;
+  int by, bz, nt;  // by == 0: two-pass form, tiles of nt cells in rows of up to bz
   size_t lds;
   void (*fn)(const SimDev<T>*, SimState*, T*, const K1Item*, int);
+  void (*fpot)(const SimDev<T>*, const SimState*, const K1Item*, int);
+  void (*fflux)(const SimDev<T>*, const SimState*, T*, const K1Item*, int);
+  bool split() const { return by == 0; }
 };
 
 template <typename T, int BY, int BZ, int NT, int MINW>
 static K1Variant<T> make_variant() {
-  return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT, MINW>};
+  return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT, MINW>, nullptr, nullptr};
+}
+template <typename T, int NT, int MINW>
+static K1Variant<T> make_split_variant() {
+  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT>};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -520,13 +539,14 @@ static std::vector<K1Variant<T>> k1_variants();
 template <>
 std::vector<K1Variant<double>> k1_variants<double>() {
   return {make_variant<double, 8, 32, 256, 2>(), make_variant<double, 4, 32, 128, 2>(),
-          make_variant<double, 8, 16, 128, 3>()};
+          make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
   return {make_variant<float, 8, 32, 256, 3>(), make_variant<float, 8, 32, 256, 2>(),
           make_variant<float, 16, 32, 256, 2>(), make_variant<float, 8, 64, 256, 2>(),
-          make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>()};
+          make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>(),
+          make_split_variant<float, 256, 3>(), make_split_variant<float, 256, 4>()};
 }
 
 constexpr int K2_NT = 256;
@@ -540,6 +560,7 @@ struct Engine : EngineBase {
     pf5d_sim_desc d{};
     long ncell = 0;
     T *X = nullptr, *Xlo = nullptr, *dX = nullptr, *psi = nullptr, *s = nullptr, *bhat = nullptr;
+    T *mu = nullptr, *mb = nullptr, *mb1 = nullptr;  // two-pass stencil intermediates
     cx<T>* C = nullptr;
     const pf5d_table* tab = nullptr;
     size_t bytes = 0;
@@ -563,7 +584,7 @@ struct Engine : EngineBase {
   bool dirty = true;
   hipGraphExec_t g1 = nullptr, gN = nullptr;
   static constexpr int GN = 8;
-  unsigned k1_grid = 0;
+  unsigned k1_grid = 0, kf_grid = 0;
 
   Engine(hipStream_t s) {
     if (s) {
@@ -572,10 +593,12 @@ struct Engine : EngineBase {
       HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       own_stream = true;
     }
-    for (auto& v : variants) (void)hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds);
+    for (auto& v : variants)
+      if (!v.split())
+        (void)hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds);
     variant = -1;
     for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
-      if (variants[i].lds <= (size_t)devinfo().lds) variant = i;
+      if (!variants[i].split() && variants[i].lds <= (size_t)devinfo().lds) variant = i;
     require(variant >= 0, "no stencil tiling fits this device's LDS");
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
@@ -600,6 +623,9 @@ struct Engine : EngineBase {
     dfree(s.s);
     dfree(s.bhat);
     dfree(s.C);
+    dfree(s.mu);
+    dfree(s.mb);
+    dfree(s.mb1);
     s = Sim();
   }
   void drop_graphs() {
@@ -655,7 +681,10 @@ struct Engine : EngineBase {
     try {
       s.X = (T*)dmalloc(ncell * 4 * sizeof(T));
       s.dX = (T*)dmalloc(ncell * 4 * sizeof(T));
-      s.bytes = ncell * 8 * sizeof(T);
+      s.mu = (T*)dmalloc(ncell * 4 * sizeof(T));
+      s.mb = (T*)dmalloc(ncell * 4 * sizeof(T));
+      s.mb1 = (T*)dmalloc(ncell * sizeof(T));
+      s.bytes = ncell * 17 * sizeof(T);
       if (sizeof(T) == 4) {
         s.Xlo = (T*)dmalloc(ncell * 4 * sizeof(T));
         s.bytes += ncell * 4 * sizeof(T);
@@ -747,6 +776,9 @@ struct Engine : EngineBase {
     v.dX = s.dX;
     v.psi = s.psi;
     v.s = s.s;
+    v.mu = s.mu;
+    v.mb = s.mb;
+    v.mb1 = s.mb1;
     v.ctld = d.ctld;
     v.dt_cfl = d.dt_cfl;
     v.lower = d.lower_limit;
@@ -772,9 +804,19 @@ struct Engine : EngineBase {
       devs[i] = make_dev(s);
       const SimDev<T>& v = devs[i];
       const bool hx = v.dim == 3;
+      if (V.split()) {
+        // tiles of at most nt cells: rows of up to bz cells (nt in 1D), as many rows as fit
+        // x fastest: a block's contiguous range of items walks down one tile column
+        const int tzmax = (v.n1 == 1) ? V.nt : V.bz;
+        for (int z0 = 0; z0 < v.n2; z0 += tzmax) {
+          const int tz = std::min(tzmax, v.n2 - z0), ry = V.nt / tz;
+          for (int y0 = 0; y0 < v.n1; y0 += ry)
+            for (int x = 0; x < v.n0; ++x) k1.push_back({i, x, x + 1, y0, z0, std::min(ry, v.n1 - y0), tz});
+        }
+      }
       const int by = (v.dim >= 2) ? V.by : 1;
       const int cxs = hx ? std::max(1, std::min(xchunk, v.n0)) : 1;
-      for (int xa = 0; xa < (hx ? v.n0 : 1); xa += cxs)
+      for (int xa = 0; xa < (hx ? v.n0 : 1) && !V.split(); xa += cxs)
         for (int y0 = 0; y0 < v.n1; y0 += by)
           for (int z0 = 0; z0 < v.n2; z0 += V.bz) {
             K1Item it;
@@ -816,9 +858,23 @@ struct Engine : EngineBase {
     }
     // K1 grid: persistent, at most what fits on the chip at once
     int occ = 1;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
-    occ = std::max(1, occ);
-    k1_grid = (unsigned)std::max(1, std::min(nk1, devinfo().cus * occ));
+    if (V.split()) {
+      // grids in multiples of the XCD count (block_range pairs blocks per XCD)
+      auto grid = [&](int o) {
+        int g = std::min(nk1, devinfo().cus * std::max(1, o));
+        if (g >= NXCD) g -= g % NXCD;
+        return (unsigned)std::max(1, g);
+      };
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fpot, V.nt, 0);
+      k1_grid = grid(occ);
+      occ = 1;
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fflux, V.nt, 0);
+      kf_grid = grid(occ);
+    } else {
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
+      occ = std::max(1, occ);
+      k1_grid = (unsigned)std::max(1, std::min(nk1, devinfo().cus * occ));
+    }
     // prime after every rebuild (the partial-sum buffer is new): misfit and FP64
     // statistics of the current X; X is not modified, so a rebuild between two
     // steps leaves every simulation's trajectory unchanged
@@ -844,6 +900,13 @@ struct Engine : EngineBase {
   void launch_k1() {
     if (!nk1) return;
     const K1Variant<T>& V = variants[variant];
+    if (V.split()) {
+      hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
+      HIPCHK(hipGetLastError());
+      hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+      HIPCHK(hipGetLastError());
+      return;
+    }
     require(V.lds <= (size_t)devinfo().lds, "K1 tile needs more LDS than a block may use");
     hipLaunchKernelGGL(V.fn, dim3(k1_grid), dim3(V.nt), V.lds, stream, d_sims, d_state, d_xmax, d_k1, nk1);
     HIPCHK(hipGetLastError());
@@ -974,10 +1037,13 @@ struct Engine : EngineBase {
     return (int)variants.size();
   }
   // average duration of one launch of a single stage, timed with HIP events on
-  // the engine stream: 0 = K1 stencil, 1 = FFT stages (all five), 2 = DT, 3 = K2 prime
+  // the engine stream: 0 = K1 stencil, 1 = FFT stages (all five), 2 = DT, 3 = K2 prime,
+  // 4 / 5 = potentials / flux pass of the two-pass stencil
   double time_kernel(int which, int reps) override {
-    require(reps > 0 && which >= 0 && which <= 3, "time_kernel: bad arguments");
+    require(reps > 0 && which >= 0 && which <= 5, "time_kernel: bad arguments");
     if (dirty) rebuild();
+    const K1Variant<T>& V = variants[variant];
+    require(which < 4 || V.split(), "time_kernel: the two-pass stencil is not selected");
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -985,7 +1051,11 @@ struct Engine : EngineBase {
       if (which == 0) launch_k1();
       else if (which == 1) fft.run(stream, d_state);
       else if (which == 2) launch_dt(0);
-      else launch_k2(1);
+      else if (which == 3) launch_k2(1);
+      else if (which == 4)
+        hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
+      else
+        hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
     };
     one();
     HIPCHK(hipEventRecord(a, stream));

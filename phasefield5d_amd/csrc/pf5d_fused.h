@@ -42,6 +42,9 @@ struct SimDev {
   T* dX;
   const T* psi;
   T* s;
+  T* mu;              // two-pass stencil: mu (ncell x 4), Mob (ncell x 4 + ncell)
+  T* mb;
+  T* mb1;
   double ctld, dt_cfl, lower, upper, t_end;
   int oob_interval;   // bounds check every N steps (reference: 100)
   double tol_soft, tol_hard;

@@ -295,6 +295,7 @@ PF5D_DEV NodeBox stage_node_box(const TableView<T>& tab, int v[8], T* box, int* 
   nb.st[1] = ext[3] * ext[2];
   nb.st[0] = nb.st[1] * ext[1];
   const int num = tab.num;
+#pragma unroll 1
   for (int e = tid; e < nb.vol * NCOL; e += NT) {
     const int node = e / NCOL, col = e - node * NCOL;
     const int q0 = node / nb.st[0], r0 = node - q0 * nb.st[0];
@@ -322,12 +323,17 @@ PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
   }
   double best = 1e300;
   long bidx = -1;
+  // rare path (outside the linear region): kept rolled so that it does not
+  // raise the register budget of the kernels it is inlined into
+#pragma unroll 1
   for (int a = 0; a < 4; ++a) {
     int n0 = lo[0] + a;
     if (n0 < 0 || n0 >= num) continue;
+#pragma unroll 1
     for (int b = 0; b < 4; ++b) {
       int n1 = lo[1] + b;
       if (n1 < 0 || n1 >= num || n0 + n1 > num - 1) continue;
+#pragma unroll 1
       for (int c = 0; c < 4; ++c) {
         int n2 = lo[2] + c;
         if (n2 < 0 || n2 >= num || n0 + n1 + n2 > num - 1) continue;

@@ -1,0 +1,247 @@
+// Two-pass form of the stencil stage: the same arithmetic as k_stencil,
+// operation for operation, so both forms give the same bits.
+//
+//   KP  k_potentials : per cell: interpolation (LDS node box) + kappa lap(X)
+//                      + elastic psi -> mu (4 values) and Mob (5 values) in HBM
+//   KF  k_fluxdiv    : per cell: the face fluxes to its neighbours from
+//                      (X, mu, Mob) -> dX/dt, max |dX/dt| per simulation
+//
+// The single-pass form keeps mu and Mob in LDS but recomputes them on a halo
+// and synchronises the block several times per plane; this form moves 9
+// values per cell through HBM twice and needs one block barrier per tile (the
+// node box), so many more waves stay resident.  Which is faster depends on
+// the shape mix; the autotuner times both.
+//
+// Work items are (sim, x plane, y0, z0, ty, tz) tiles of at most NT cells:
+// thread t handles cell (x, y0 + t / tz, z0 + t % tz).  Items are listed with
+// x fastest and every block takes a contiguous range of them, so a block walks
+// consecutive planes of one tile and finds its x neighbours in cache.  A face flux is
+// evaluated by both cells it separates, always with the lower cell first, so
+// both see the same value (the divergence telescopes exactly, as in the
+// single-pass form where each face is evaluated once).
+#pragma once
+#include "pf5d_common.h"
+#include "pf5d_flux.h"
+#include "pf5d_fused.h"
+#include "pf5d_interp.h"
+
+namespace pf5d {
+
+template <typename T>
+struct SplitSmem {
+  static constexpr int BOXMAX = sizeof(T) == 4 ? 256 : 128;
+};
+
+// XCDs (dies with their own L2) of the chip; workgroups are handed to them
+// round robin, workgroup b to XCD b % NXCD
+constexpr int NXCD = 8;
+
+// This block's contiguous item range [ib, ie).  Consecutive ranges go to
+// blocks on the same XCD, so the neighbouring planes a block reads were
+// brought into that XCD's L2 by the blocks working next to it.
+PF5D_DEV void block_range(int nitems, int& ib, int& ie) {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int L = (G % NXCD == 0) ? (b % NXCD) * (G / NXCD) + b / NXCD : b;
+  const int per = (nitems + G - 1) / G;
+  ib = min(nitems, L * per);
+  ie = min(nitems, ib + per);
+}
+
+// cell index of the tile thread, and whether it has a cell
+template <typename T>
+PF5D_DEV bool tile_cell(const SimDev<T>& sd, const K1Item& item, int tid, float rtz, int& gx, int& gy, int& gz) {
+  int j, k;
+  fdivmod(tid, item.tz, rtz, j, k);
+  gx = item.xa;
+  gy = item.y0 + j;
+  gz = item.z0 + k;
+  return j < item.ty;
+}
+
+// MINW: minimum waves per SIMD the register allocation must allow
+template <typename T, int NT, int MINW>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
+k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st, const K1Item* __restrict__ items,
+             int nitems) {
+  constexpr int BOXMAX = SplitSmem<T>::BOXMAX;
+  constexpr int NP = TablePad<T>::value;
+  __shared__ __align__(16) T BOX[BOXMAX * NP];
+  __shared__ int BRED[(NT / 64) * 8 + 8];
+  int ib, ie;
+  block_range(nitems, ib, ie);
+  for (int it = ib; it < ie; ++it) {
+    const K1Item item = items[it];
+    if (!(st[item.sim].status & ST_ACTIVE)) continue;  // uniform per block
+    // an empty asm keeps address arithmetic from being hoisted out of the item
+    // loop (holding it across items costs more registers than recomputing it)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const SimDev<T>& sd = sims[item.sim];
+    const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
+    const bool hx = sd.dim == 3, hy = sd.dim >= 2;
+    int gx, gy, gz;
+    const bool valid = tile_cell<T>(sd, item, tid, 1.0f / (float)item.tz, gx, gy, gz);
+    const long c = ((long)gx * n1 + gy) * n2 + gz;
+    V4<T> xc{};
+    if (valid) xc = ld4g<T>(sd.X + 4 * c);
+    int v[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) v[a] = 0x7fffffff;
+    if (valid && ((xc.v[0] + xc.v[1]) + xc.v[2]) + xc.v[3] <= sd.tab.thresh) node_box_point<T>(sd.tab, xc.v, v);
+    const NodeBox nb = stage_node_box<T, NT, BOXMAX>(sd.tab, v, BOX, BRED, tid);
+    if (valid) {
+      T P[9];
+      interp9_staged<T>(sd.tab, BOX, nb, xc.v, P);
+      // neighbour sums in the reference order: x, then y, then z (each up + down)
+      T acc[4];
+      bool first = true;
+      if (hx) {
+        const long s = (long)n1 * n2;
+        const V4<T> u = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? s : s - (long)n0 * s)));
+        const V4<T> d = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -s : (long)(n0 - 1) * s)));
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a] = u.v[a] + d.v[a];
+        first = false;
+      }
+      if (hy) {
+        const long u_i = c + (long)(wrap(gy + 1, n1) - gy) * n2, d_i = c + (long)(wrap(gy - 1, n1) - gy) * n2;
+        const V4<T> u = ld4g<T>(sd.X + 4 * u_i), d = ld4g<T>(sd.X + 4 * d_i);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
+        first = false;
+      }
+      {
+        const V4<T> u = ld4g<T>(sd.X + 4 * (c + (wrap(gz + 1, n2) - gz)));
+        const V4<T> d = ld4g<T>(sd.X + 4 * (c + (wrap(gz - 1, n2) - gz)));
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
+      }
+      const T psi = sd.elastic ? gld<T>(sd.psi + c) : T(0);
+      const T cdim = T(2 * sd.dim);
+      V4<T> mu, mb;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        // (acc - 2 dim x) / h^2, then P - L kappa (+ psi l): simulate.py:253-255
+        const T L = mad<T>(-cdim, xc.v[a], acc[a]) * sd.inv_dx2;
+        T m = mad<T>(-L, sd.kappa[a], P[a]);
+        if (sd.elastic) m = mad<T>(psi, sd.l[a], m);
+        mu.v[a] = m;
+        mb.v[a] = P[4 + a];
+      }
+      st4g<T>(sd.mu + 4 * c, mu);
+      st4g<T>(sd.mb + 4 * c, mb);
+      gst<T>(sd.mb1 + c, P[8]);
+    }
+    __syncthreads();  // BOX / BRED are reused by the next item
+  }
+}
+
+// X, mu and Mob of one cell
+template <typename T>
+struct CellV {
+  V4<T> X, mu;
+  T M[5];
+};
+
+template <typename T>
+PF5D_DEV CellV<T> load_cell(const SimDev<T>& sd, long c) {
+  CellV<T> r;
+  r.X = ld4g<T>(sd.X + 4 * c);
+  r.mu = ld4g<T>(sd.mu + 4 * c);
+  const V4<T> m = ld4g<T>(sd.mb + 4 * c);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) r.M[a] = m.v[a];
+  r.M[4] = gld<T>(sd.mb1 + c);
+  return r;
+}
+
+// flux through the face from cell a (lower) to cell b (upper)
+template <typename T>
+PF5D_DEV void face_ab(const CellV<T>& a, const CellV<T>& b, T inv_dx, T out[4]) {
+  T g[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) g[k] = (b.mu.v[k] - a.mu.v[k]) * inv_dx;
+  face_flux<T>(a.X.v, b.X.v, a.M, b.M, g, out);
+}
+
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st,
+                                                T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
+                                                int nitems) {
+  __shared__ T RED[NT / 64];
+  const int tid = threadIdx.x;
+  // a simulation's items are adjacent, so the block max is flushed with one
+  // atomic per simulation the block's range touches
+  int ib, ie;
+  block_range(nitems, ib, ie);
+  T tmax = T(0);
+  int cur = -1;
+  auto flush = [&](int sim) {
+    const T m = wave_max<T>(tmax);
+    if ((tid & 63) == 0) RED[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+      T r = RED[0];
+      for (int w = 1; w < NT / 64; ++w) r = (RED[w] > r) ? RED[w] : r;
+      atomic_max_nonneg(&xmax_acc[sim], r);
+    }
+    __syncthreads();
+    tmax = T(0);
+  };
+  for (int it = ib; it < ie; ++it) {
+    const K1Item item = items[it];
+    if (item.sim != cur) {
+      if (cur >= 0) flush(cur);
+      cur = item.sim;
+    }
+    if (!(st[item.sim].status & ST_ACTIVE)) continue;  // uniform per block
+    const SimDev<T>& sd = sims[item.sim];
+    const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
+    const bool hx = sd.dim == 3, hy = sd.dim >= 2;
+    int gx, gy, gz;
+    if (!tile_cell<T>(sd, item, tid, 1.0f / (float)item.tz, gx, gy, gz)) continue;
+    const long c = ((long)gx * n1 + gy) * n2 + gz;
+    const T inv_dx = sd.inv_dx;
+    const CellV<T> me = load_cell<T>(sd, c);
+    T acc[4], fp[4], fm[4];
+    bool first = true;
+    // divergence in the reference order: x, y, z, each (F+ - F-) added in turn
+    if (hx) {
+      const long s = (long)n1 * n2;
+      const CellV<T> up = load_cell<T>(sd, c + (gx + 1 < n0 ? s : s - (long)n0 * s));
+      const CellV<T> dn = load_cell<T>(sd, c + (gx > 0 ? -s : (long)(n0 - 1) * s));
+      face_ab<T>(me, up, inv_dx, fp);
+      face_ab<T>(dn, me, inv_dx, fm);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] = fp[a] - fm[a];
+      first = false;
+    }
+    if (hy) {
+      const CellV<T> up = load_cell<T>(sd, c + (long)(wrap(gy + 1, n1) - gy) * n2);
+      const CellV<T> dn = load_cell<T>(sd, c + (long)(wrap(gy - 1, n1) - gy) * n2);
+      face_ab<T>(me, up, inv_dx, fp);
+      face_ab<T>(dn, me, inv_dx, fm);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] = first ? (fp[a] - fm[a]) : ((acc[a] + fp[a]) - fm[a]);
+      first = false;
+    }
+    {
+      const CellV<T> up = load_cell<T>(sd, c + (wrap(gz + 1, n2) - gz));
+      const CellV<T> dn = load_cell<T>(sd, c + (wrap(gz - 1, n2) - gz));
+      face_ab<T>(me, up, inv_dx, fp);
+      face_ab<T>(dn, me, inv_dx, fm);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] = first ? (fp[a] - fm[a]) : ((acc[a] + fp[a]) - fm[a]);
+    }
+    V4<T> o;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      o.v[a] = acc[a] * inv_dx;
+      tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
+    }
+    st4g<T>(sd.dX + 4 * c, o);
+  }
+  if (cur >= 0) flush(cur);
+}
+
+}  // namespace pf5d

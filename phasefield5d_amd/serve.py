@@ -58,6 +58,7 @@ def footprint_bytes(pb, precision):
     w = 4 if precision == "fp32" else 8
     n = pb.ncell
     b = n * 4 * w * (3 if precision == "fp32" else 2)     # X, dX (+ remainder in FP32)
+    b += n * 9 * w                                         # mu, Mob of the two-pass stencil
     if pb.Bhat is not None:
         nh = int(np.prod(pb.Bhat.shape))
         b += n * 2 * w + nh * 3 * w                        # psi, s, spectrum (complex), Bhat

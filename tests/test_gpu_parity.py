@@ -127,23 +127,31 @@ def test_elastic_update(dev, dim, shape):
     assert rel_inf(mu - mu0, ref - mu0) < TOL64
 
 
-def _engine_one_step(pb, precision="fp64"):
+def _split_variant(eng):
+    """Index of the two-pass stencil form among the engine's tilings (by == 0)."""
+    return next(i for i, (by, bz, nt) in enumerate(eng.variants()) if by == 0)
+
+
+def _engine_one_step(pb, precision="fp64", form="fused"):
     from phasefield5d_amd.engine import Engine
     eng = Engine(precision)
     slot = eng.add_problem(pb)
+    if form == "split":
+        eng.set_tiling(_split_variant(eng), 1)
     eng.step(1)
     eng.sync()
     return eng, slot
 
 
+@pytest.mark.parametrize("form", ["fused", "split"])
 @pytest.mark.parametrize("dim,n,elastic", [(1, 160, True), (2, 32, True), (2, 32, False), (3, 16, True),
                                            (3, 24, False)])
-def test_fused_step_fp64(dev, dim, n, elastic):
+def test_fused_step_fp64(dev, dim, n, elastic, form):
     pb = problem(dim, n, elastic=elastic)
     pb.X_init = developed_state(pb)
     ora = oracle_sim(pb)
     Xn, dX, dt_new, _, xmax = ora.step(pb.X_init, pb.dt0)
-    eng, slot = _engine_one_step(pb)
+    eng, slot = _engine_one_step(pb, form=form)
     st = eng.state(slot)
     dXg = eng.field(slot, "dXdt")
     if elastic:
@@ -220,6 +228,32 @@ def test_retiling_midrun_keeps_trajectory(dev, precision):
     assert st["status"] & ST_ACTIVE and st["steps"] == 6
     assert np.array_equal(a.field(sa, "X"), b.field(sb, "X"))
     assert np.allclose(st["mass"], pb.X_init.reshape(-1, 4).sum(axis=0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_stencil_forms_bitwise(dev, precision):
+    """Every stencil tiling, single-pass or two-pass, gives the same bits: rows
+    longer than a tile (1D 600, 2D 96 = 64 + 32), ragged 3D tiles (20^3), with
+    and without elasticity, in one batch."""
+    from phasefield5d_amd.engine import Engine
+    pbs = [problem(1, 600, seed=5), problem(2, 96, seed=6), problem(3, 20, seed=7), problem(2, 40, seed=8,
+                                                                                           elastic=False)]
+    for p in pbs:
+        p.X_init = developed_state(p, amp=0.03)
+    out = {}
+    probe = Engine(precision)
+    nvar = len(probe.variants())
+    probe.close()
+    for v in range(nvar):
+        e = Engine(precision)
+        slots = [e.add_problem(p) for p in pbs]
+        e.set_tiling(v, 4)
+        e.step(3)
+        out[v] = [(e.field(s, "X"), e.field(s, "dXdt"), e.state(s)["dt"]) for s in slots]
+        e.close()
+    for v in range(1, nvar):
+        for (Xa, da, ta), (Xb, db, tb) in zip(out[0], out[v]):
+            assert np.array_equal(Xa, Xb) and np.array_equal(da, db) and ta == tb, f"variant {v}"
 
 
 def test_batched_equals_alone(dev):
