@@ -41,7 +41,7 @@ def _store(cache):
         pass
 
 
-def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft_budgets=(8, 16, 32, 64)):
+def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft_budgets=(8, 16, 32, 64, 160)):
     """Pick the fastest (variant, x-chunk) and FFT batching for the engine's current simulations."""
     sig = signature(engine)
     with _lock:
@@ -76,7 +76,7 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
     if any(pb.Bhat is not None for pb, _ in engine.slots.values()):
         fres = []
         for kb in fft_budgets:
-            for nt in (128, 256):
+            for nt in (128, 256, 512):
                 try:
                     engine.set_fft(kb, nt)
                     fms = engine.time_kernel("fft", reps=reps)

@@ -1,13 +1,21 @@
-"""Build libpf5d.so in-tree with hipcc for gfx950 (no JIT, no cache outside the tree)."""
+"""Build libpf5d.so in-tree with hipcc for gfx950 (no JIT, no cache outside the tree).
+
+The library is four translation units compiled in parallel -- the host unit
+(C ABI, engine, small kernels), the FFT kernels in FP32 and in FP64, and the
+stencil kernels -- linked into one shared object.
+"""
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "pf5d_capi.hip")
+CSRC = os.path.join(HERE, "csrc")
+UNITS = ["pf5d_capi.hip", "pf5d_kern_fft32.hip", "pf5d_kern_fft64.hip", "pf5d_kern_stencil.hip"]
 OUT = os.path.join(HERE, "libpf5d.so")
+OBJDIR = os.path.join(HERE, "_obj")
 
-HIPCC_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+HIPCC_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC",
                # FP64 kernels keep the reference's separate roundings; FP32 kernels fuse explicitly
                "-ffp-contract=off",
                # no packed FP32 instructions
@@ -18,17 +26,28 @@ def hipcc():
     return os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
-def build(verbose=False):
-    tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp, SRC]
+def _run(cmd, what):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode}) building {SRC}")
-    if verbose:
-        msgs = [l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l and l.strip()]
-        if msgs:
-            print("\n".join(msgs))
+        raise RuntimeError(f"hipcc failed ({r.returncode}) building {what}")
+    return [l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l and l.strip()]
+
+
+def build(verbose=False):
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = [os.path.join(OBJDIR, u.replace(".hip", ".o")) for u in UNITS]
+
+    def compile_unit(i):
+        src = os.path.join(CSRC, UNITS[i])
+        return _run([hipcc()] + HIPCC_FLAGS + ["-c", "-o", objs[i], src], src)
+
+    with ThreadPoolExecutor(max_workers=len(UNITS)) as ex:
+        msgs = sum(ex.map(compile_unit, range(len(UNITS))), [])
+    tmp = OUT + ".tmp"
+    msgs += _run([hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + objs, OUT)
+    if verbose and msgs:
+        print("\n".join(msgs))
     os.replace(tmp, OUT)
     return OUT
 

@@ -19,6 +19,13 @@
 #include "pf5d_fused.h"
 #include "pf5d_interp.h"
 #include "pf5d_split.h"
+#include "pf5d_instances.h"
+
+namespace pf5d {
+PF5D_FFT32_INSTANCES
+PF5D_FFT64_INSTANCES
+PF5D_STENCIL_INSTANCES
+}  // namespace pf5d
 
 using namespace pf5d;
 
@@ -93,6 +100,13 @@ static DevInfo& devinfo() {
 
 static inline unsigned nblocks(long n, int bs) { return (unsigned)std::max<long>(1, (n + bs - 1) / bs); }
 
+// grid size rounded up to a multiple of the XCD count, so that block_range
+// gives each XCD one contiguous share of a work list (extra blocks get none)
+static unsigned xcd_grid(long g) {
+  g = std::max<long>(1, g);
+  return (unsigned)((g + NXCD - 1) / NXCD * NXCD);
+}
+
 // library-internal grow-only device scratch for reductions
 struct Scratch {
   void* p = nullptr;
@@ -145,33 +159,10 @@ static void upload_table(pf5d_table* t, const double* grid, int nprops) {
 // ---------------------------------------------------------------------------
 // FFT plans and batched pencil passes
 // ---------------------------------------------------------------------------
+// the FFT plan of a line length (the radix order the kernels assume)
 static std::vector<int> factorize(int n) {
   std::vector<int> r;
-  int m = n;
-  while (m % 8 == 0 && m >= 8) {
-    r.push_back(8);
-    m /= 8;
-  }
-  while (m % 4 == 0) {
-    r.push_back(4);
-    m /= 4;
-  }
-  while (m % 2 == 0) {
-    r.push_back(2);
-    m /= 2;
-  }
-  for (int p : {3, 5, 7}) {
-    while (m % p == 0) {
-      r.push_back(p);
-      m /= p;
-    }
-  }
-  for (int p = 11; m > 1; p += 2) {
-    while (m % p == 0) {
-      r.push_back(p);
-      m /= p;
-    }
-  }
+  for (int p = 0; fft_radix(n, p) != 0; ++p) r.push_back(fft_radix(n, p));
   return r;
 }
 
@@ -198,12 +189,6 @@ struct FftPlans {
     if (it != index.end()) return it->second;
     std::vector<int> rad = factorize(n);
     require((int)rad.size() <= FFT_MAXPASS, "FFT length has too many factors: " + std::to_string(n));
-    if (ct_length(n)) {
-      // the kernel's compile-time plan for this length must be this factorisation
-      bool same = ct_npass(n) == (int)rad.size();
-      for (int p = 0; same && p < (int)rad.size(); ++p) same = ct_radix(n, p) == rad[p];
-      require(same, "compile-time FFT plan differs from the host plan for n = " + std::to_string(n));
-    }
     LinePlanDev pl{};
     pl.n = n;
     pl.npass = (int)rad.size();
@@ -254,6 +239,23 @@ struct FftPlans {
   }
 };
 
+// Rows of the half spectrum (2D and 3D) are padded to whole 128-byte lines, so
+// that groups of lines along the strided axes start on cache-line boundaries
+// and neighbouring work items never share a line (the padding is not read).
+template <typename T>
+static long spec_row(int dim, const long* shape) {
+  const long nh = shape[dim - 1] / 2 + 1;
+  if (dim == 1) return nh;
+  const long a = 128 / (long)sizeof(cx<T>);
+  return (nh + a - 1) / a * a;
+}
+template <typename T>
+static long spectrum_size(int dim, const long* shape) {
+  long rows = 1;
+  for (int a = 0; a < dim - 1; ++a) rows *= shape[a];
+  return rows * spec_row<T>(dim, shape);
+}
+
 // the five stages of psi = irfftn(Bhat rfftn(s)) and their work items
 template <typename T>
 struct FftBatch {
@@ -269,22 +271,26 @@ struct FftBatch {
   int nt = 128;
 
   static bool paired(int kind) { return kind == FK_R2C || kind == FK_C2R; }
-  // lines per item: two line buffers within the LDS budget (real-line stages
-  // carry two lines per complex line)
+  // complex lines per item: two line buffers within the LDS budget (real-line
+  // stages carry two lines per complex line)
   int pick_g(int n, long nlines, int kind) const {
-    long g = (long)(budget / (2 * (size_t)line_stride(n) * sizeof(cx<T>)));
-    g = std::max<long>(1, std::min<long>(g, 64));
+    const size_t avail = budget > (size_t)n * sizeof(cx<T>) ? budget - (size_t)n * sizeof(cx<T>) : 0;
+    long g = (long)(avail / (2 * (size_t)line_stride(n) * sizeof(cx<T>)));
+    g = std::max<long>(1, std::min<long>(g, 128));
     if (paired(kind)) g *= 2;
     return (int)std::min<long>(g, nlines);
   }
   void push(int st, FftItem it, int n) {
-    stage[st].push_back(it);
     const size_t gl = paired(it.kind) ? (it.G + 1) / 2 : it.G;
+    stage[st].push_back(it);
     lds[st] = std::max(lds[st], ((size_t)n + 2 * gl * line_stride(n)) * sizeof(cx<T>));
   }
-  // lines [0, nlines) with line l at base + l*ls; split into groups of G
-  void add_lines(int st, FftItem proto, int n, long nlines, long line_unit_in, long line_unit_out, long line_unit_b) {
-    const int G = pick_g(n, nlines, proto.kind);
+  // lines [0, nlines) with line l at base + l*ls; split into groups of G (a
+  // multiple of align lines when G >= align)
+  void add_lines(int st, FftItem proto, int n, long nlines, long line_unit_in, long line_unit_out, long line_unit_b,
+                 int align = 1) {
+    int G = pick_g(n, nlines, proto.kind);
+    if (G >= align) G -= G % align;
     for (long l0 = 0; l0 < nlines; l0 += G) {
       FftItem it = proto;
       it.G = (int)std::min<long>(G, nlines - l0);
@@ -298,7 +304,8 @@ struct FftBatch {
   void add_sim(int sim, int dim, const long* shape, const T* s, cx<T>* C, T* psi,
                const T* bhat) {
     const int nl = (int)shape[dim - 1];
-    const long nh = nl / 2 + 1;
+    const long nh = nl / 2 + 1, nhp = spec_row<T>(dim, shape);
+    const int al = 128 / (int)sizeof(cx<T>);  // lines per 128-byte segment along a strided axis
     const long rows = [&] {
       long r = 1;
       for (int a = 0; a < dim - 1; ++a) r *= shape[a];
@@ -328,17 +335,17 @@ struct FftBatch {
     b.ps_in = 1;
     b.ls_in = nl;
     b.ps_out = 1;
-    b.ls_out = nh;
-    add_lines(0, b, nl, rows, sizeof(T) * nl, sizeof(cx<T>) * nh, 0);
+    b.ls_out = nhp;
+    add_lines(0, b, nl, rows, sizeof(T) * nl, sizeof(cx<T>) * nhp, 0);
     // rows: C2R
     b.kind = FK_C2R;
     b.in = C;
     b.out = psi;
     b.ps_in = 1;
-    b.ls_in = nh;
+    b.ls_in = nhp;
     b.ps_out = 1;
     b.ls_out = nl;
-    add_lines(4, b, nl, rows, sizeof(cx<T>) * nh, sizeof(T) * nl, 0);
+    add_lines(4, b, nl, rows, sizeof(cx<T>) * nhp, sizeof(T) * nl, 0);
     if (dim == 2) {
       const int nx = (int)shape[0];
       b.kind = FK_CONV;
@@ -346,35 +353,38 @@ struct FftBatch {
       b.in = C;
       b.out = C;
       b.bhat = bhat;
-      b.ps_in = b.ps_out = b.ps_b = nh;
+      b.ps_in = b.ps_out = b.ps_b = nhp;
       b.ls_in = b.ls_out = b.ls_b = 1;
-      add_lines(2, b, nx, nh, sizeof(cx<T>), sizeof(cx<T>), sizeof(T));
+      add_lines(2, b, nx, nh, sizeof(cx<T>), sizeof(cx<T>), sizeof(T), al);
       return;
     }
     const int nx = (int)shape[0], ny = (int)shape[1];
-    // y passes: for each x plane, lines kz in [0, nh) with point stride nh
+    // y passes: for each x plane, lines kz in [0, nh) with point stride nhp
     for (int x = 0; x < nx; ++x) {
       FftItem y{};
       y.sim = sim;
       y.plan = plans.get(ny);
-      y.in = C + (long)x * ny * nh;
-      y.out = C + (long)x * ny * nh;
-      y.ps_in = y.ps_out = nh;
+      y.in = C + (long)x * ny * nhp;
+      y.out = C + (long)x * ny * nhp;
+      y.ps_in = y.ps_out = nhp;
       y.ls_in = y.ls_out = 1;
       y.kind = FK_FWD;
-      add_lines(1, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0);
+      add_lines(1, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
       y.kind = FK_INV;
-      add_lines(3, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0);
+      add_lines(3, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
     }
-    // x pass with Bhat: lines (y, kz) flattened, point stride ny*nh
-    b.kind = FK_CONV;
-    b.plan = plans.get(nx);
-    b.in = C;
-    b.out = C;
-    b.bhat = bhat;
-    b.ps_in = b.ps_out = b.ps_b = (long)ny * nh;
-    b.ls_in = b.ls_out = b.ls_b = 1;
-    add_lines(2, b, nx, (long)ny * nh, sizeof(cx<T>), sizeof(cx<T>), sizeof(T));
+    // x pass with Bhat: for each y, lines kz in [0, nh) with point stride ny*nhp
+    for (int y = 0; y < ny; ++y) {
+      FftItem c = b;
+      c.kind = FK_CONV;
+      c.plan = plans.get(nx);
+      c.in = C + (long)y * nhp;
+      c.out = C + (long)y * nhp;
+      c.bhat = bhat + (long)y * nhp;
+      c.ps_in = c.ps_out = c.ps_b = (long)ny * nhp;
+      c.ls_in = c.ls_out = c.ls_b = 1;
+      add_lines(2, c, nx, nh, sizeof(cx<T>), sizeof(cx<T>), sizeof(T), al);
+    }
   }
   void clear() {
     for (auto& s : stage) s.clear();
@@ -399,8 +409,14 @@ struct FftBatch {
       const int n = offs[i + 1] - offs[i];
       if (n == 0) continue;
       require(lds[i] <= (size_t)di.lds, "FFT line too long for LDS: " + std::to_string(lds[i]) + " bytes");
-      const unsigned grid = (unsigned)std::min<long>(n, (long)di.cus * 16);
-      if (nt == 256)
+      const unsigned grid = xcd_grid(std::min<long>(n, (long)di.cus * 32));
+      if (nt == 64)
+        hipLaunchKernelGGL((k_fft<T, 64>), dim3(grid), dim3(64), lds[i], stream, d_items + offs[i], n,
+                           plans.d_plans, plans.d_tw, plans.d_roots, st);
+      else if (nt == 512)
+        hipLaunchKernelGGL((k_fft<T, 512>), dim3(grid), dim3(512), lds[i], stream, d_items + offs[i], n,
+                           plans.d_plans, plans.d_tw, plans.d_roots, st);
+      else if (nt == 256)
         hipLaunchKernelGGL((k_fft<T, 256>), dim3(grid), dim3(256), lds[i], stream, d_items + offs[i], n,
                            plans.d_plans, plans.d_tw, plans.d_roots, st);
       else
@@ -415,21 +431,17 @@ struct FftBatch {
 // Bhat (k_shape, double) -> device T in the spectrum layout, with 1/N folded in
 template <typename T>
 static T* upload_bhat(int dim, const long* shape, const double* bhat_host) {
-  long nk = 1, ncell = 1;
+  long rows = 1, ncell = 1;
   for (int a = 0; a < dim; ++a) ncell *= shape[a];
-  for (int a = 0; a < dim - 1; ++a) nk *= shape[a];
-  nk *= shape[dim - 1] / 2 + 1;
-  std::vector<T> h(nk);
+  for (int a = 0; a < dim - 1; ++a) rows *= shape[a];
+  const long nh = shape[dim - 1] / 2 + 1, nhp = spec_row<T>(dim, shape);
+  std::vector<T> h(rows * nhp, T(0));
   const double inv = 1.0 / (double)ncell;
-  for (long i = 0; i < nk; ++i) h[i] = (T)(bhat_host[i] * inv);
-  T* d = (T*)dmalloc(nk * sizeof(T));
-  HIPCHK(hipMemcpy(d, h.data(), nk * sizeof(T), hipMemcpyHostToDevice));
+  for (long r = 0; r < rows; ++r)
+    for (long k = 0; k < nh; ++k) h[r * nhp + k] = (T)(bhat_host[r * nh + k] * inv);
+  T* d = (T*)dmalloc(h.size() * sizeof(T));
+  HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   return d;
-}
-static long spectrum_size(int dim, const long* shape) {
-  long nk = 1;
-  for (int a = 0; a < dim - 1; ++a) nk *= shape[a];
-  return nk * (shape[dim - 1] / 2 + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -454,7 +466,7 @@ template <typename T>
 static void elastic_init(pf5d_elastic* e, const double* bhat_host) {
   e->s = dmalloc(e->ncell * sizeof(T));
   e->psi = dmalloc(e->ncell * sizeof(T));
-  e->C = dmalloc(spectrum_size(e->dim, e->shape) * sizeof(cx<T>));
+  e->C = dmalloc(spectrum_size<T>(e->dim, e->shape) * sizeof(cx<T>));
   e->bhat = upload_bhat<T>(e->dim, e->shape, bhat_host);
   auto b = std::make_unique<FftBatch<T>>();
   b->add_sim(0, e->dim, e->shape, (const T*)e->s, (cx<T>*)e->C, (T*)e->psi, (const T*)e->bhat);
@@ -528,9 +540,11 @@ template <typename T, int BY, int BZ, int NT, int MINW>
 static K1Variant<T> make_variant() {
   return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT, MINW>, nullptr, nullptr};
 }
-template <typename T, int NT, int MINW>
+// MINW: register budget of the potentials pass; EAGER: the flux pass loads all
+// seven cells up front
+template <typename T, int NT, int MINW, bool EAGER>
 static K1Variant<T> make_split_variant() {
-  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT>};
+  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, EAGER>};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -539,14 +553,16 @@ static std::vector<K1Variant<T>> k1_variants();
 template <>
 std::vector<K1Variant<double>> k1_variants<double>() {
   return {make_variant<double, 8, 32, 256, 2>(), make_variant<double, 4, 32, 128, 2>(),
-          make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2>()};
+          make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, false>(),
+          make_split_variant<double, 256, 2, true>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
   return {make_variant<float, 8, 32, 256, 3>(), make_variant<float, 8, 32, 256, 2>(),
           make_variant<float, 16, 32, 256, 2>(), make_variant<float, 8, 64, 256, 2>(),
           make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>(),
-          make_split_variant<float, 256, 3>(), make_split_variant<float, 256, 4>()};
+          make_split_variant<float, 256, 3, false>(), make_split_variant<float, 256, 4, false>(),
+          make_split_variant<float, 256, 4, true>()};
 }
 
 constexpr int K2_NT = 256;
@@ -600,6 +616,8 @@ struct Engine : EngineBase {
     for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
       if (!variants[i].split() && variants[i].lds <= (size_t)devinfo().lds) variant = i;
     require(variant >= 0, "no stencil tiling fits this device's LDS");
+    (void)hipFuncSetAttribute((const void*)&k_fft<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+    (void)hipFuncSetAttribute((const void*)&k_fft<T, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
   }
@@ -692,9 +710,9 @@ struct Engine : EngineBase {
       if (d->elastic) {
         s.psi = (T*)dmalloc(ncell * sizeof(T));
         s.s = (T*)dmalloc(ncell * sizeof(T));
-        s.C = (cx<T>*)dmalloc(spectrum_size(d->dim, d->shape) * sizeof(cx<T>));
+        s.C = (cx<T>*)dmalloc(spectrum_size<T>(d->dim, d->shape) * sizeof(cx<T>));
         s.bhat = upload_bhat<T>(d->dim, d->shape, bhat);
-        s.bytes += ncell * 2 * sizeof(T) + spectrum_size(d->dim, d->shape) * (sizeof(cx<T>) + sizeof(T));
+        s.bytes += ncell * 2 * sizeof(T) + spectrum_size<T>(d->dim, d->shape) * (sizeof(cx<T>) + sizeof(T));
       }
       upload_X(s, Xh);
       HIPCHK(hipMemset(s.dX, 0, ncell * 4 * sizeof(T)));
@@ -806,12 +824,18 @@ struct Engine : EngineBase {
       const bool hx = v.dim == 3;
       if (V.split()) {
         // tiles of at most nt cells: rows of up to bz cells (nt in 1D), as many rows as fit
-        // x fastest: a block's contiguous range of items walks down one tile column
+        // slabs of planes (one per XCD, in the order the other kernels' lists
+        // take them), then tile columns, x fastest: a block's contiguous range
+        // of items walks down one tile column of one slab
         const int tzmax = (v.n1 == 1) ? V.nt : V.bz;
-        for (int z0 = 0; z0 < v.n2; z0 += tzmax) {
-          const int tz = std::min(tzmax, v.n2 - z0), ry = V.nt / tz;
-          for (int y0 = 0; y0 < v.n1; y0 += ry)
-            for (int x = 0; x < v.n0; ++x) k1.push_back({i, x, x + 1, y0, z0, std::min(ry, v.n1 - y0), tz});
+        const int nsl = std::min(NXCD, v.n0);
+        for (int sl = 0; sl < nsl; ++sl) {
+          const int xa = sl * v.n0 / nsl, xb = (sl + 1) * v.n0 / nsl;
+          for (int z0 = 0; z0 < v.n2; z0 += tzmax) {
+            const int tz = std::min(tzmax, v.n2 - z0), ry = V.nt / tz;
+            for (int y0 = 0; y0 < v.n1; y0 += ry)
+              for (int x = xa; x < xb; ++x) k1.push_back({i, x, x + 1, y0, z0, std::min(ry, v.n1 - y0), tz});
+          }
         }
       }
       const int by = (v.dim >= 2) ? V.by : 1;
@@ -859,12 +883,7 @@ struct Engine : EngineBase {
     // K1 grid: persistent, at most what fits on the chip at once
     int occ = 1;
     if (V.split()) {
-      // grids in multiples of the XCD count (block_range pairs blocks per XCD)
-      auto grid = [&](int o) {
-        int g = std::min(nk1, devinfo().cus * std::max(1, o));
-        if (g >= NXCD) g -= g % NXCD;
-        return (unsigned)std::max(1, g);
-      };
+      auto grid = [&](int o) { return xcd_grid(std::min(nk1, devinfo().cus * std::max(1, o))); };
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fpot, V.nt, 0);
       k1_grid = grid(occ);
       occ = 1;
@@ -887,7 +906,7 @@ struct Engine : EngineBase {
   }
   void launch_k2(int prime) {
     if (!nk2) return;
-    const unsigned grid = (unsigned)std::min(nk2, devinfo().cus * 8);
+    const unsigned grid = xcd_grid(std::min(nk2, devinfo().cus * 8));
     hipLaunchKernelGGL((k_update<T, K2_NT>), dim3(grid), dim3(K2_NT), 0, stream, d_sims, d_state, d_k2, nk2, d_part,
                        prime);
     HIPCHK(hipGetLastError());
@@ -1022,7 +1041,7 @@ struct Engine : EngineBase {
   }
   void set_fft(int budget_kb, int nt) override {
     require(budget_kb >= 4 && budget_kb <= 160, "FFT LDS budget must be 4..160 KiB");
-    require(nt == 128 || nt == 256, "FFT block size must be 128 or 256");
+    require(nt == 64 || nt == 128 || nt == 256 || nt == 512, "FFT block size must be 64, 128, 256 or 512");
     fft.budget = (size_t)budget_kb * 1024;
     fft.nt = nt;
     dirty = true;

@@ -120,6 +120,23 @@ PF5D_DEV double wave_sum(double v) {
   return v;
 }
 
+// XCDs (dies with their own L2) of the chip; workgroups are handed to them
+// round robin, workgroup b to XCD b % NXCD
+constexpr int NXCD = 8;
+
+// This block's contiguous item range [ib, ie) (the items split as evenly as
+// possible).  With a grid that is a multiple of NXCD, XCD j takes the j-th
+// eighth of the list: work lists ordered by region (planes of a simulation)
+// keep each region on one XCD from kernel to kernel, so what one kernel wrote
+// is in the L2 the next one reads, and a block's neighbouring planes were
+// brought in by the blocks working next to it.
+PF5D_DEV void block_range(int nitems, int& ib, int& ie) {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int L = (G % NXCD == 0) ? (b % NXCD) * (G / NXCD) + b / NXCD : b;
+  ib = (int)((long)L * nitems / G);
+  ie = (int)((long)(L + 1) * nitems / G);
+}
+
 enum StatusBits { ST_ACTIVE = 1, ST_HARD_OOB = 2, ST_SOFT_OOB = 4, ST_DONE = 8 };
 
 // per-simulation mutable state (FP64 scalars on the device)

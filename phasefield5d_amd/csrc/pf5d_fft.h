@@ -6,13 +6,20 @@
 // on graphics processors", SC08: for each pass with radix R and sub-transform
 // size Ns, butterfly j reads x[j + r n/R], twiddles by exp(-2 pi i r (j mod Ns)
 // / (Ns R)), applies a radix-R DFT and writes y[(j/Ns) Ns R + (j mod Ns) + r Ns].
-// Radices 2, 3, 4, 5, 8 are hand-written butterflies; any other factor uses a
-// direct radix-R DFT from a per-plan root table.  Twiddles are computed on the
-// host in extended precision and rounded to the working type.
+// Radices 2, 3, 4, 5, 8, 16 are hand-written butterflies; any other factor uses
+// a direct radix-R DFT from a per-plan root table.  Twiddles are computed on
+// the host in extended precision and rounded to the working type.  Plans split
+// the power-of-two part of n into as few passes of radix <= 16 as possible,
+// balanced (128 = 16 x 8, 512 = 8 x 8 x 8), then the odd factors.
 //
 // Every work item transforms G lines of length n.  Point m of line g sits at
 // g*ls + m*ps (ps = 1 for rows along the contiguous axis, ls = 1 for pencils
 // along a strided axis, so global loads are coalesced either way).
+//
+// Lengths with a compile-time plan run their first pass straight from global
+// memory and their last pass straight to it (registers in between), so a
+// two-pass line costs one LDS exchange; other lengths run every pass through
+// LDS with the plan read at run time.
 //
 // Inverse transforms use IFFT(x) = conj(FFT(conj(x))).  The 1/N of irfftn is
 // folded into Bhat on the host.  Because Bhat is real and even in k, the
@@ -120,6 +127,74 @@ PF5D_DEV void dft5(cx<T>* v) {
   v[2] = cadd(b2, md2);
   v[3] = csub(b2, md2);
 }
+// 16 = 4 x 4: X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[4 n1 + n2] W4^(n1 k1)
+template <typename T>
+PF5D_DEV void dft16(cx<T>* v) {
+  const T c1 = T(0.92387953251128675612818318939678828682);  // cos(pi/8)
+  const T s1 = T(0.38268343236508977172845998403039886676);  // sin(pi/8)
+  const T r2 = T(0.70710678118654752440084436210484903928);
+  cx<T> y[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    cx<T> a0 = v[n2], a1 = v[4 + n2], a2 = v[8 + n2], a3 = v[12 + n2];
+    dft4(a0, a1, a2, a3);
+    y[n2 * 4 + 0] = a0;
+    y[n2 * 4 + 1] = a1;
+    y[n2 * 4 + 2] = a2;
+    y[n2 * 4 + 3] = a3;
+  }
+  // twiddles W16^(n2 k1) = exp(-2 pi i n2 k1 / 16)
+  y[5] = cmul(y[5], cx<T>{c1, -s1});    // W^1
+  y[6] = cmul(y[6], cx<T>{r2, -r2});    // W^2
+  y[7] = cmul(y[7], cx<T>{s1, -c1});    // W^3
+  y[9] = cmul(y[9], cx<T>{r2, -r2});    // W^2
+  y[10] = mul_minus_i(y[10]);           // W^4
+  y[11] = cmul(y[11], cx<T>{-r2, -r2});  // W^6
+  y[13] = cmul(y[13], cx<T>{s1, -c1});  // W^3
+  y[14] = cmul(y[14], cx<T>{-r2, -r2});  // W^6
+  y[15] = cmul(y[15], cx<T>{-c1, s1});  // W^9
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cx<T> b0 = y[k1], b1 = y[4 + k1], b2 = y[8 + k1], b3 = y[12 + k1];
+    dft4(b0, b1, b2, b3);
+    v[k1] = b0;
+    v[k1 + 4] = b1;
+    v[k1 + 8] = b2;
+    v[k1 + 12] = b3;
+  }
+}
+
+// Plan: radix of pass p for length n (0 past the last pass).  The power-of-two
+// part 2^k takes ceil(k/4) passes with the bits spread evenly (larger radices
+// first), then the odd prime factors in increasing order.
+PF5D_HD constexpr int fft_radix(int n, int p) {
+  int k = 0, m = n;
+  while (m % 2 == 0 && m > 1) {
+    m /= 2;
+    ++k;
+  }
+  const int P2 = (k + 3) / 4;
+  if (p < P2) return 1 << (k / P2 + (p < k % P2 ? 1 : 0));
+  int i = P2;
+  for (int q = 3; m > 1; q += 2) {
+    while (m % q == 0) {
+      if (i == p) return q;
+      m /= q;
+      ++i;
+    }
+  }
+  return 0;
+}
+PF5D_HD constexpr int fft_npass(int n) {
+  int p = 0;
+  while (fft_radix(n, p) != 0) ++p;
+  return p;
+}
+PF5D_HD constexpr int fft_min_radix(int n) {
+  int r = 1 << 30;
+  for (int p = 0; p < fft_npass(n); ++p) r = fft_radix(n, p) < r ? fft_radix(n, p) : r;
+  return r;
+}
 
 // LDS position of point e of a line: one spare slot after every 8 points, so
 // the power-of-two strides of the Stockham reads and writes spread over the
@@ -143,7 +218,15 @@ PF5D_DEV void stockham_pass(const cx<T>* __restrict__ src, cx<T>* __restrict__ d
     cx<T>* d = dst + g * LS;
     const int ob = q * Ns * R + k;
     const cx<T>* w = tw + k * (R - 1);  // w[r-1] is the twiddle of input r >= 1
-    if (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) {
+    if (R == 16) {
+      cx<T> v[16];
+      v[0] = s[lpad(j)];
+#pragma unroll
+      for (int r = 1; r < 16; ++r) v[r] = (Ns == 1) ? s[lpad(j + r * nb)] : cmul(s[lpad(j + r * nb)], w[r - 1]);
+      dft16(v);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) d[lpad(ob + r * Ns)] = v[r];
+    } else if (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) {
       cx<T> v[8];
       v[0] = s[lpad(j)];
 #pragma unroll
@@ -198,32 +281,68 @@ PF5D_DEV cx<T>* fft_lines(cx<T>* buf0, cx<T>* buf1, const LinePlanDev& pl, int L
 }
 
 // ---------------------------------------------------------------------------
-// Compile-time plans for the line lengths of the usual grids: the radix list
-// (the host's factorisation order: 8s, 4s, 2s, then 3, 5, 7) is fixed at
-// compile time, so every index computation of a pass folds to shifts and
-// constant multiplies and the butterfly of each pass is a straight-line block.
+// Item input and output, shared by both paths
 // ---------------------------------------------------------------------------
-PF5D_HD constexpr int ct_radix(int n, int p) {
-  int m = n, i = 0;
-  const int order[6] = {8, 4, 2, 3, 5, 7};
-  for (int q = 0; q < 6; ++q) {
-    const int R = order[q];
-    while (m % R == 0 && m >= R) {
-      if (i == p) return R;
-      m /= R;
-      ++i;
-    }
+// Point m of complex line g of an item as the forward transform consumes it:
+// R2C packs real lines 2g, 2g+1 as real and imaginary parts; C2R builds
+// conj(A + i B) from the Hermitian extensions of half spectra 2g, 2g+1
+// (conj(A_full[m]) = conj(A[m]) for m < nh, A[n - m] above); INV conjugates.
+template <typename T>
+PF5D_DEV cx<T> item_input(const FftItem& I, int G, int n, int nh, int g, int m) {
+  if (I.kind == FK_R2C) {
+    const T* in = reinterpret_cast<const T*>(I.in);
+    const int g0 = 2 * g, g1 = 2 * g + 1;
+    return {gld<T>(in + (g0 * I.ls_in + (long)m * I.ps_in)),
+            g1 < G ? gld<T>(in + (g1 * I.ls_in + (long)m * I.ps_in)) : T(0)};
   }
-  return 0;
+  if (I.kind == FK_C2R) {
+    const cx<T>* C = reinterpret_cast<const cx<T>*>(I.in);
+    auto cfull = [&](int l) -> cx<T> {
+      const cx<T>* L = C + l * I.ls_in;
+      return (m < nh) ? conjg(gld<cx<T>>(L + (long)m * I.ps_in)) : gld<cx<T>>(L + (long)(n - m) * I.ps_in);
+    };
+    const cx<T> ca = cfull(2 * g);
+    const cx<T> cb = (2 * g + 1 < G) ? cfull(2 * g + 1) : cx<T>{T(0), T(0)};
+    return {ca.x + cb.y, ca.y - cb.x};
+  }
+  if (I.kind == FK_LINE) return {gld<T>(reinterpret_cast<const T*>(I.in) + (g * I.ls_in + (long)m * I.ps_in)), T(0)};
+  const cx<T> v = gld<cx<T>>(reinterpret_cast<const cx<T>*>(I.in) + (g * I.ls_in + (long)m * I.ps_in));
+  return (I.kind == FK_INV) ? conjg(v) : v;
 }
-PF5D_HD constexpr int ct_npass(int n) {
-  int p = 0;
-  while (ct_radix(n, p) != 0) ++p;
-  return p;
+
+// Bhat factor of point m of line g (CONV and LINE items)
+template <typename T>
+PF5D_DEV T item_bhat(const FftItem& I, int n, int nh, int g, int m) {
+  if (I.kind == FK_LINE) return gld<T>(reinterpret_cast<const T*>(I.bhat) + ((m < nh) ? m : n - m));
+  return gld<T>(reinterpret_cast<const T*>(I.bhat) + (g * I.ls_b + (long)m * I.ps_b));
 }
-// lengths with a compile-time plan
+
+// Final point m of complex line g (not R2C): C2R writes Re to real line 2g and
+// -Im to line 2g+1 (z = conj(W)); LINE the real part; FWD as is; INV and CONV
+// the conjugate.
+template <typename T>
+PF5D_DEV void item_output(const FftItem& I, int G, int g, int m, const cx<T>& v) {
+  if (I.kind == FK_C2R) {
+    T* out = reinterpret_cast<T*>(I.out);
+    gst<T>(out + (2 * g * I.ls_out + (long)m * I.ps_out), v.x);
+    if (2 * g + 1 < G) gst<T>(out + ((2 * g + 1) * I.ls_out + (long)m * I.ps_out), -v.y);
+  } else if (I.kind == FK_LINE) {
+    gst<T>(reinterpret_cast<T*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), v.x);
+  } else {
+    const cx<T> o = (I.kind == FK_FWD) ? v : conjg(v);
+    gst<cx<T>>(reinterpret_cast<cx<T>*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), o);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time path: lengths whose plan is known when the kernel is built.
+// Two LDS buffers.  Along a strided axis (consecutive lines adjacent in
+// memory) the first pass reads global memory and the last pass writes it
+// directly, with the lines varying fastest across threads; rows along the
+// contiguous axis are gathered into LDS and stored from it, point-contiguous.
+// ---------------------------------------------------------------------------
 PF5D_HD constexpr bool ct_length(int n) {
-  return n == 64 || n == 96 || n == 128 || n == 160 || n == 192 || n == 256 || n == 384 || n == 512;
+  return n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 512;
 }
 
 template <int R, typename T>
@@ -236,207 +355,241 @@ PF5D_DEV void dft_r(cx<T>* v) {
     dft4(v[0], v[1], v[2], v[3]);
   else if constexpr (R == 5)
     dft5(v);
-  else
+  else if constexpr (R == 8)
     dft8(v);
+  else
+    dft16(v);
 }
 
-// one pass with radix R after sub-transforms of size NS, lines of N points
-template <typename T, int NT, int N, int R, int NS>
-PF5D_DEV void stockham_ct(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int G,
-                          const cx<T>* __restrict__ tw, int tid) {
-  constexpr int NB = N / R, LS = line_stride(N);
-  const int tot = G * NB;
+// One radix-R pass after sub-transforms of size NS over GL lines of N points.
+// Butterfly b maps to (line g, index j) line-major, or point-major (pm: the
+// lines vary fastest, for global accesses along a strided axis).  Inputs
+// src(g, j + r N/R); outputs dst(g, (j/NS) NS R + j%NS + r NS, value).
+template <typename T, int NT, int N, int R, int NS, typename Src, typename Dst>
+PF5D_DEV void ct_pass(int GL, float rgl, int tid, const cx<T>* __restrict__ tw, bool pm, Src src, Dst dst) {
+  constexpr int NB = N / R;
+  const int tot = GL * NB;
 #pragma unroll 1
   for (int b = tid; b < tot; b += NT) {
-    const int g = b / NB, j = b - g * NB;
-    const int k = j % NS, q = j / NS;
-    const cx<T>* s = src + g * LS;
-    cx<T>* d = dst + g * LS + 0;
+    int g, j;
+    if (pm) {
+      fdivmod(b, GL, rgl, j, g);
+    } else {
+      g = b / NB;
+      j = b - g * NB;
+    }
     cx<T> v[R];
-    v[0] = s[lpad(j)];
 #pragma unroll
-    for (int r = 1; r < R; ++r) {
-      const cx<T> x = s[lpad(j + r * NB)];
-      if constexpr (NS == 1)
-        v[r] = x;
-      else
-        v[r] = cmul(x, tw[k * (R - 1) + r - 1]);
+    for (int r = 0; r < R; ++r) v[r] = src(g, j + r * NB);
+    const int k = j % NS;
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[k * (R - 1) + r - 1]);
     }
     dft_r<R>(v);
-    const int ob = q * NS * R + k;
+    const int ob = (j / NS) * NS * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) d[lpad(ob + r * NS)] = v[r];
+    for (int r = 0; r < R; ++r) dst(g, ob + r * NS, v[r]);
   }
 }
 
-// the passes of the compile-time plan; tw holds this pass's twiddles first
-// (pass p's twiddles start at NS_p - 1 in the plan's table)
+// Passes P.. of the plan for N, alternating between buffers b0 and b1: pass 0
+// reads fetch and writes b0, pass p > 0 reads the buffer pass p-1 wrote; the
+// last pass hands its outputs to sink(g, m, v, other) where other is the
+// buffer it does not read.  Pass P's twiddles start at NS - 1 in tw.
 template <typename T, int NT, int N, int P, int NS>
-struct CtPasses {
-  static PF5D_DEV cx<T>* run(cx<T>* a, cx<T>* b, int G, const cx<T>* tw, int tid) {
-    if constexpr (P == ct_npass(N)) {
-      return a;
+struct CtRun {
+  template <typename Fetch, typename Sink>
+  static PF5D_DEV void run(int GL, float rgl, int tid, const cx<T>* tw, cx<T>* b0, cx<T>* b1, bool pm_src,
+                           Fetch fetch, bool pm_dst, Sink sink) {
+    constexpr int NP = fft_npass(N), R = fft_radix(N, P), LS = line_stride(N);
+    cx<T>* in = (P % 2 == 1) ? b0 : b1;   // written by pass P-1
+    cx<T>* out = (P % 2 == 0) ? b0 : b1;  // written by this pass
+    auto lds_src = [&](int g, int m) -> cx<T> { return in[g * LS + lpad(m)]; };
+    auto lds_dst = [&](int g, int m, const cx<T>& v) { out[g * LS + lpad(m)] = v; };
+    const cx<T>* w = tw + (NS - 1);
+    if constexpr (P == NP - 1) {
+      auto last = [&](int g, int m, const cx<T>& v) { sink(g, m, v, out); };
+      if constexpr (P == 0)
+        ct_pass<T, NT, N, R, NS>(GL, rgl, tid, w, pm_dst, fetch, last);
+      else
+        ct_pass<T, NT, N, R, NS>(GL, rgl, tid, w, pm_dst, lds_src, last);
     } else {
-      constexpr int R = ct_radix(N, P);
-      stockham_ct<T, NT, N, R, NS>(a, b, G, tw, tid);
+      if constexpr (P == 0)
+        ct_pass<T, NT, N, R, NS>(GL, rgl, tid, w, pm_src, fetch, lds_dst);
+      else
+        ct_pass<T, NT, N, R, NS>(GL, rgl, tid, w, false, lds_src, lds_dst);
       __syncthreads();
-      return CtPasses<T, NT, N, P + 1, NS * R>::run(b, a, G, tw + NS * (R - 1), tid);
+      CtRun<T, NT, N, P + 1, NS * R>::run(GL, rgl, tid, tw, b0, b1, pm_src, fetch, pm_dst, sink);
     }
   }
 };
 
-// q = idx / n, r = idx % n with n = N when N > 0 (compile time), else runtime
-template <int N>
-PF5D_DEV void divmod_n(int idx, int n, float rn, int& q, int& r) {
-  if constexpr (N > 0) {
-    q = idx / N;
-    r = idx - q * N;
-  } else {
-    fdivmod(idx, n, rn, q, r);
-  }
-}
-
-// Work item layout in LDS: the plan's n - 1 twiddles (n slots), then two line
-// buffers of GL lines (stride line_stride(n)).  R2C and C2R items carry their
-// G real lines as GL = ceil(G/2) complex lines (line 2q in the real part, line
-// 2q+1 in the imaginary part), which halves their transform work.
-// N > 0: compile-time length N; N = 0: the plan's length at run time.
+// LDS: the plan's twiddles (N slots), then two buffers of GL lines
 template <typename T, int NT, int N>
-PF5D_DEV void fft_item(const FftItem& I, const LinePlanDev& pl, const cx<T>* __restrict__ tw,
-                       const cx<T>* __restrict__ roots, unsigned char* smem) {
-  // the thread index is passed through an empty asm statement, so that address
-  // arithmetic derived from it is not hoisted out of the caller's item loop
-  // (holding it across items would cost more registers than recomputing it)
-  int tid = threadIdx.x;
-  asm volatile("" : "+v"(tid));
-  const int n = N > 0 ? N : pl.n;
-  const int G = I.G, nh = n / 2 + 1, LS = line_stride(n);
+PF5D_DEV void fft_item_ct(const FftItem& I, const cx<T>* twl, cx<T>* b0, int tid) {
+  constexpr int LS = line_stride(N), NH = N / 2 + 1;
+  const int G = I.G;
   const bool paired = (I.kind == FK_R2C || I.kind == FK_C2R);
   const int GL = paired ? (G + 1) / 2 : G;
-  cx<T>* twl = reinterpret_cast<cx<T>*>(smem);
-  cx<T>* buf0 = twl + n;
-  cx<T>* buf1 = buf0 + GL * LS;
-  const float rn = 1.0f / (float)n, rg = 1.0f / (float)G;
-  // ---- load
-  if (paired) {
-    // rows along the contiguous axis (point stride 1); pair q = lines 2q, 2q+1
-    const int totq = GL * n;
-    #pragma unroll 1
-    for (int idx = tid; idx < totq; idx += NT) {
-      int q, m;
-      divmod_n<N>(idx, n, rn, q, m);
-      const int g0 = 2 * q, g1 = 2 * q + 1;
-      const bool has1 = g1 < G;
-      cx<T> v;
-      if (I.kind == FK_R2C) {
-        const T* in = reinterpret_cast<const T*>(I.in);
-        v.x = gld<T>(in + (g0 * I.ls_in + (long)m * I.ps_in));
-        v.y = has1 ? gld<T>(in + (g1 * I.ls_in + (long)m * I.ps_in)) : T(0);
-      } else {
-        // Hermitian extension of a half spectrum, conjugated for the inverse:
-        // conj(A_full[m]) = conj(A[m]) (m < nh) or A[n - m]; then conj(A + i B) = conj(A) - i conj(B)
-        const cx<T>* C = reinterpret_cast<const cx<T>*>(I.in);
-        auto cfull = [&](int g) -> cx<T> {
-          const cx<T>* L = C + g * I.ls_in;
-          return (m < nh) ? conjg(gld<cx<T>>(L + (long)m * I.ps_in)) : gld<cx<T>>(L + (long)(n - m) * I.ps_in);
-        };
-        const cx<T> ca = cfull(g0);
-        const cx<T> cb = has1 ? cfull(g1) : cx<T>{T(0), T(0)};
-        v = {ca.x + cb.y, ca.y - cb.x};
-      }
-      buf0[q * LS + lpad(m)] = v;
+  cx<T>* b1 = b0 + GL * LS;
+  const float rgl = 1.0f / (float)GL;
+  // strided lines read and write global memory from the first / last pass
+  const bool fuse_in = I.ps_in != 1, fuse_out = I.ps_out != 1;
+  auto fetch_global = [&](int g, int m) -> cx<T> { return item_input<T>(I, G, N, NH, g, m); };
+  // rows: gathered point-contiguous into b1 first (the first pass writes b0)
+  auto fetch_b1 = [&](int g, int m) -> cx<T> { return b1[g * LS + lpad(m)]; };
+  if (!fuse_in) {
+#pragma unroll 1
+    for (int idx = tid; idx < GL * N; idx += NT) {
+      const int g = idx / N, m = idx - g * N;
+      b1[g * LS + lpad(m)] = item_input<T>(I, G, N, NH, g, m);
     }
+    __syncthreads();
+  }
+  auto natural = [&](int g, int m, const cx<T>& v, cx<T>* other) { other[g * LS + lpad(m)] = v; };
+  // final values: straight to global memory (strided) or through LDS (rows)
+  auto finish = [&](int g, int m, const cx<T>& v, cx<T>* other) {
+    if (fuse_out)
+      item_output<T>(I, G, g, m, v);
+    else
+      other[g * LS + lpad(m)] = v;
+  };
+  // the buffer the last pass's sink writes (the one it does not read): pass p
+  // writes b0 when p is even, so the last pass of an odd-length plan writes b0
+  constexpr bool last_in_b0 = (fft_npass(N) % 2 == 1);
+  cx<T>* res = last_in_b0 ? b0 : b1;
+  if (I.kind == FK_CONV || I.kind == FK_LINE) {
+    // forward; W = conj(Z) * Bhat in natural order; forward again (from that buffer)
+    auto sink_bhat = [&](int g, int m, const cx<T>& z, cx<T>* other) {
+      const T b = item_bhat<T>(I, N, NH, g, m);
+      other[g * LS + lpad(m)] = {z.x * b, -z.y * b};
+    };
+    if (fuse_in)
+      CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b0, b1, true, fetch_global, false, sink_bhat);
+    else
+      CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b0, b1, false, fetch_b1, false, sink_bhat);
+    __syncthreads();
+    // W sits in res; the second transform's first pass reads it
+    cx<T>* W = res;
+    auto fetch_w = [&](int g, int m) -> cx<T> { return W[g * LS + lpad(m)]; };
+    if (W == b0) {
+      // the first pass must not write the buffer it reads: run with the roles swapped
+      CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b1, b0, false, fetch_w, fuse_out, finish);
+      res = last_in_b0 ? b1 : b0;
+    } else {
+      CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b0, b1, false, fetch_w, fuse_out, finish);
+    }
+  } else if (I.kind == FK_R2C) {
+    CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b0, b1, false, fetch_b1, false, natural);
+  } else if (fuse_in) {
+    CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b0, b1, true, fetch_global, fuse_out, finish);
   } else {
-    // point-major order for strided lines: consecutive threads take consecutive lines
-    const bool rowmajor = (I.ps_in == 1);
-    const int tot = G * n;
-    #pragma unroll 1
-    for (int idx = tid; idx < tot; idx += NT) {
-      int g, m;
-      if (rowmajor)
-        divmod_n<N>(idx, n, rn, g, m);
-      else
-        fdivmod(idx, G, rg, m, g);
-      cx<T> v;
-      if (I.kind == FK_LINE) {
-        v = {gld<T>(reinterpret_cast<const T*>(I.in) + (g * I.ls_in + (long)m * I.ps_in)), T(0)};
-      } else {
-        v = gld<cx<T>>(reinterpret_cast<const cx<T>*>(I.in) + (g * I.ls_in + (long)m * I.ps_in));
-        if (I.kind == FK_INV) v = conjg(v);
-      }
-      buf0[g * LS + lpad(m)] = v;
+    CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b0, b1, false, fetch_b1, fuse_out, finish);
+  }
+  if (I.kind == FK_R2C) {
+    // Z = FFT(a + i b): A[k] = (Z[k] + conj(Z[-k])) / 2, B[k] = -i (Z[k] - conj(Z[-k])) / 2, k < nh
+    __syncthreads();
+    const int totp = G * NH;
+#pragma unroll 1
+    for (int idx = tid; idx < totp; idx += NT) {
+      const int g = idx / NH, m = idx - g * NH;
+      const cx<T>* Z = res + (g >> 1) * LS;
+      const cx<T> z = Z[lpad(m)], zr = Z[lpad(m == 0 ? 0 : N - m)];
+      const cx<T> o = (g & 1) ? cx<T>{T(0.5) * (z.y + zr.y), T(0.5) * (zr.x - z.x)}
+                              : cx<T>{T(0.5) * (z.x + zr.x), T(0.5) * (z.y - zr.y)};
+      gst<cx<T>>(reinterpret_cast<cx<T>*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), o);
+    }
+  } else if (!fuse_out) {
+    __syncthreads();
+#pragma unroll 1
+    for (int idx = tid; idx < GL * N; idx += NT) {
+      const int g = idx / N, m = idx - g * N;
+      item_output<T>(I, G, g, m, res[g * LS + lpad(m)]);
     }
   }
   __syncthreads();
-  auto transform = [&](cx<T>* a, cx<T>* b) -> cx<T>* {
-    if constexpr (N > 0)
-      return CtPasses<T, NT, N, 0, 1>::run(a, b, GL, twl, tid);
-    else
-      return fft_lines<T, NT>(a, b, pl, LS, GL, twl, roots);
-  };
-  cx<T>* res = transform(buf0, buf1);
-  if (I.kind == FK_CONV || I.kind == FK_LINE) {
-    // W = conj(Z) * Bhat, then a second forward transform; result = conj(V)
-    const int tl = GL * n;
-    #pragma unroll 1
-    for (int idx = tid; idx < tl; idx += NT) {
-      int g, m;
-      divmod_n<N>(idx, n, rn, g, m);
-      T b;
-      if (I.kind == FK_LINE)
-        b = gld<T>(reinterpret_cast<const T*>(I.bhat) + ((m < nh) ? m : n - m));
-      else
-        b = gld<T>(reinterpret_cast<const T*>(I.bhat) + (g * I.ls_b + (long)m * I.ps_b));
-      cx<T>& z = res[g * LS + lpad(m)];
-      z = {z.x * b, -z.y * b};
+}
+
+// ---------------------------------------------------------------------------
+// Run-time path: any length, every pass LDS to LDS (two buffers)
+// ---------------------------------------------------------------------------
+// dst[slot(idx)] = fetch(idx) for idx = tid, tid + NT, ... < tot, FFT_LB at a
+// time: all fetches of a batch are issued before the first LDS write
+constexpr int FFT_LB = 4;
+template <typename T, int NT, typename F>
+PF5D_DEV void gather_lds(int tot, int tid, cx<T>* dst, F fetch) {
+#pragma unroll 1
+  for (int base = tid; base < tot; base += NT * FFT_LB) {
+    cx<T> v[FFT_LB];
+    int at[FFT_LB];
+#pragma unroll
+    for (int u = 0; u < FFT_LB; ++u) {
+      at[u] = -1;
+      if (base + u * NT < tot) v[u] = fetch(base + u * NT, at[u]);
     }
-    __syncthreads();
-    res = transform(res, (res == buf0) ? buf1 : buf0);
+#pragma unroll
+    for (int u = 0; u < FFT_LB; ++u)
+      if (at[u] >= 0) dst[at[u]] = v[u];
   }
-  // ---- store
-  if (paired) {
-    if (I.kind == FK_R2C) {
-      const int totp = G * nh;
-      const float rnh = 1.0f / (float)nh;
-      #pragma unroll 1
-      for (int idx = tid; idx < totp; idx += NT) {
-        int g, m;
-        divmod_n<(N > 0 ? N / 2 + 1 : 0)>(idx, nh, rnh, g, m);
-        // Z = FFT(a + i b): A[k] = (Z[k] + conj(Z[-k])) / 2, B[k] = -i (Z[k] - conj(Z[-k])) / 2
-        const cx<T>* Z = res + (g >> 1) * LS;
-        const cx<T> z = Z[lpad(m)], zr = Z[lpad(m == 0 ? 0 : n - m)];
-        const cx<T> o = (g & 1) ? cx<T>{T(0.5) * (z.y + zr.y), T(0.5) * (zr.x - z.x)}
-                                : cx<T>{T(0.5) * (z.x + zr.x), T(0.5) * (z.y - zr.y)};
-        gst<cx<T>>(reinterpret_cast<cx<T>*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), o);
-      }
-    } else {
-      const int totp = G * n;
-      #pragma unroll 1
-      for (int idx = tid; idx < totp; idx += NT) {
-        int g, m;
-        divmod_n<N>(idx, n, rn, g, m);
-        // z = conj(W): line 2q is Re W, line 2q+1 is -Im W
-        const cx<T> w = res[(g >> 1) * LS + lpad(m)];
-        gst<T>(reinterpret_cast<T*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), (g & 1) ? -w.y : w.x);
-      }
-    }
-  } else {
-    const bool rowmajor_o = (I.ps_out == 1);
-    const int totp = G * n;
-    #pragma unroll 1
+}
+
+// LDS: the plan's twiddles (n slots), then two buffers of GL lines
+template <typename T, int NT>
+PF5D_DEV void fft_item_rt(const FftItem& I, const LinePlanDev& pl, const cx<T>* twl, const cx<T>* __restrict__ roots,
+                          cx<T>* buf0, int tid) {
+  const int n = pl.n, G = I.G, nh = n / 2 + 1, LS = line_stride(n);
+  const bool paired = (I.kind == FK_R2C || I.kind == FK_C2R);
+  const int GL = paired ? (G + 1) / 2 : G;
+  cx<T>* buf1 = buf0 + GL * LS;
+  const float rn = 1.0f / (float)n, rgl = 1.0f / (float)GL;
+  const bool pm_in = I.ps_in != 1, pm_out = I.ps_out != 1;
+  gather_lds<T, NT>(GL * n, tid, buf0, [&](int idx, int& dst) -> cx<T> {
+    int g, m;
+    if (pm_in)
+      fdivmod(idx, GL, rgl, m, g);
+    else
+      fdivmod(idx, n, rn, g, m);
+    dst = g * LS + lpad(m);
+    return item_input<T>(I, G, n, nh, g, m);
+  });
+  __syncthreads();
+  cx<T>* res = fft_lines<T, NT>(buf0, buf1, pl, LS, GL, twl, roots);
+  if (I.kind == FK_CONV || I.kind == FK_LINE) {
+    gather_lds<T, NT>(GL * n, tid, res, [&](int idx, int& dst) -> cx<T> {
+      int g, m;
+      fdivmod(idx, n, rn, g, m);
+      dst = g * LS + lpad(m);
+      const T b = item_bhat<T>(I, n, nh, g, m);
+      const cx<T> z = res[dst];
+      return {z.x * b, -z.y * b};
+    });
+    __syncthreads();
+    res = fft_lines<T, NT>(res, (res == buf0) ? buf1 : buf0, pl, LS, GL, twl, roots);
+  }
+  if (I.kind == FK_R2C) {
+    const int totp = G * nh;
+    const float rnh = 1.0f / (float)nh;
+#pragma unroll 1
     for (int idx = tid; idx < totp; idx += NT) {
       int g, m;
-      if (rowmajor_o)
-        divmod_n<N>(idx, n, rn, g, m);
+      fdivmod(idx, nh, rnh, g, m);
+      const cx<T>* Z = res + (g >> 1) * LS;
+      const cx<T> z = Z[lpad(m)], zr = Z[lpad(m == 0 ? 0 : n - m)];
+      const cx<T> o = (g & 1) ? cx<T>{T(0.5) * (z.y + zr.y), T(0.5) * (zr.x - z.x)}
+                              : cx<T>{T(0.5) * (z.x + zr.x), T(0.5) * (z.y - zr.y)};
+      gst<cx<T>>(reinterpret_cast<cx<T>*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), o);
+    }
+  } else {
+    const int totp = GL * n;
+#pragma unroll 1
+    for (int idx = tid; idx < totp; idx += NT) {
+      int g, m;
+      if (pm_out)
+        fdivmod(idx, GL, rgl, m, g);
       else
-        fdivmod(idx, G, rg, m, g);
-      const cx<T> v = res[g * LS + lpad(m)];
-      if (I.kind == FK_LINE) {
-        gst<T>(reinterpret_cast<T*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), v.x);
-      } else {
-        const cx<T> o = (I.kind == FK_INV || I.kind == FK_CONV) ? conjg(v) : v;
-        gst<cx<T>>(reinterpret_cast<cx<T>*>(I.out) + (g * I.ls_out + (long)m * I.ps_out), o);
-      }
+        fdivmod(idx, n, rn, g, m);
+      item_output<T>(I, G, g, m, res[g * LS + lpad(m)]);
     }
   }
   __syncthreads();
@@ -447,27 +600,36 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
                                             const LinePlanDev* __restrict__ plans, const cx<T>* __restrict__ tw,
                                             const cx<T>* __restrict__ roots, const SimState* __restrict__ st) {
   extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* twl = reinterpret_cast<cx<T>*>(smem);
   int staged = -1;  // plan whose twiddles are in LDS
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  // contiguous item ranges, consecutive ranges on one XCD: a stage's items for
+  // one region of a simulation run on the XCD whose L2 holds what the previous
+  // stage wrote there
+  int ib, ie;
+  block_range(nitems, ib, ie);
+  for (int it = ib; it < ie; ++it) {
     const FftItem I = items[it];
     if (st && !(st[I.sim].status & ST_ACTIVE)) continue;
     const LinePlanDev& pl = plans[I.plan];
+    // an empty asm keeps address arithmetic derived from the thread index from
+    // being hoisted out of the item loop (it would hold registers across items)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     if (I.plan != staged) {
-      // the previous item's buffers are free: its store loop ended in a barrier
-      cx<T>* twl = reinterpret_cast<cx<T>*>(smem);
-      for (int i = threadIdx.x; i < pl.n - 1; i += NT) twl[i] = gld<cx<T>>(tw + pl.tw_off[0] + i);
+      // the previous item's buffers are free: every item ends in a barrier
+      for (int i = tid; i < pl.n - 1; i += NT) twl[i] = gld<cx<T>>(tw + pl.tw_off[0] + i);
       staged = I.plan;
+      __syncthreads();
     }
+    cx<T>* buf = twl + pl.n;
     switch (pl.n) {
-      case 64: fft_item<T, NT, 64>(I, pl, tw, roots, smem); break;
-      case 96: fft_item<T, NT, 96>(I, pl, tw, roots, smem); break;
-      case 128: fft_item<T, NT, 128>(I, pl, tw, roots, smem); break;
-      case 160: fft_item<T, NT, 160>(I, pl, tw, roots, smem); break;
-      case 192: fft_item<T, NT, 192>(I, pl, tw, roots, smem); break;
-      case 256: fft_item<T, NT, 256>(I, pl, tw, roots, smem); break;
-      case 384: fft_item<T, NT, 384>(I, pl, tw, roots, smem); break;
-      case 512: fft_item<T, NT, 512>(I, pl, tw, roots, smem); break;
-      default: fft_item<T, NT, 0>(I, pl, tw, roots, smem); break;
+      case 64: fft_item_ct<T, NT, 64>(I, twl, buf, tid); break;
+      case 96: fft_item_ct<T, NT, 96>(I, twl, buf, tid); break;
+      case 128: fft_item_ct<T, NT, 128>(I, twl, buf, tid); break;
+      case 192: fft_item_ct<T, NT, 192>(I, twl, buf, tid); break;
+      case 256: fft_item_ct<T, NT, 256>(I, twl, buf, tid); break;
+      case 512: fft_item_ct<T, NT, 512>(I, twl, buf, tid); break;
+      default: fft_item_rt<T, NT>(I, pl, twl, roots, buf, tid); break;
     }
   }
 }

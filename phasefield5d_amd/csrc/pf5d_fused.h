@@ -499,7 +499,9 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
                                                const K2Item* __restrict__ items, int nitems, double* __restrict__ part,
                                                int prime) {
   __shared__ double red[NT / 64][NSTAT];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  int ib, ie;
+  block_range(nitems, ib, ie);
+  for (int it = ib; it < ie; ++it) {
     const K2Item item = items[it];
     const SimState& S = st[item.sim];
     const bool run = prime || (S.status & ST_ACTIVE);

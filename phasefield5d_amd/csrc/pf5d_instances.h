@@ -1,0 +1,63 @@
+// The large kernels are compiled in translation units of their own
+// (pf5d_kern_fft32.hip, pf5d_kern_fft64.hip, pf5d_kern_stencil.hip), in
+// parallel with the host unit.
+// The host unit sees these instances as explicit instantiation declarations
+// (PF5D_INST = extern template), so it neither compiles nor inlines them; the
+// kernel units define them (PF5D_INST = template).
+#pragma once
+#include "pf5d_fft.h"
+#include "pf5d_fused.h"
+#include "pf5d_split.h"
+
+#ifndef PF5D_INST
+#define PF5D_INST extern template
+#endif
+
+namespace pf5d {
+
+#define PF5D_FFT_INSTANCE(T, NT)                                                                               \
+  PF5D_INST __global__ void k_fft<T, NT>(const FftItem* __restrict__, int, const LinePlanDev* __restrict__, \
+                                         const cx<T>* __restrict__, const cx<T>* __restrict__,                 \
+                                         const SimState* __restrict__);
+#define PF5D_STENCIL_INSTANCE(T, BY, BZ, NT, MINW)                                                 \
+  PF5D_INST __global__ void k_stencil<T, BY, BZ, NT, MINW>(const SimDev<T>* __restrict__,          \
+                                                           SimState* __restrict__, T* __restrict__, \
+                                                           const K1Item* __restrict__, int);
+#define PF5D_POTENTIALS_INSTANCE(T, NT, MINW)                                                              \
+  PF5D_INST __global__ void k_potentials<T, NT, MINW>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
+                                                      const K1Item* __restrict__, int);
+#define PF5D_FLUX_INSTANCE(T, NT, EAGER)                                                                       \
+  PF5D_INST __global__ void k_fluxdiv<T, NT, EAGER>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
+                                                    T* __restrict__, const K1Item* __restrict__, int);
+
+#define PF5D_FFT32_INSTANCES      \
+  PF5D_FFT_INSTANCE(float, 64)    \
+  PF5D_FFT_INSTANCE(float, 128)   \
+  PF5D_FFT_INSTANCE(float, 256)   \
+  PF5D_FFT_INSTANCE(float, 512)
+#define PF5D_FFT64_INSTANCES      \
+  PF5D_FFT_INSTANCE(double, 64)   \
+  PF5D_FFT_INSTANCE(double, 128)  \
+  PF5D_FFT_INSTANCE(double, 256)  \
+  PF5D_FFT_INSTANCE(double, 512)
+
+// the stencil tilings and two-pass forms the engine offers the autotuner
+#define PF5D_STENCIL_INSTANCES                    \
+  PF5D_STENCIL_INSTANCE(double, 8, 32, 256, 2)    \
+  PF5D_STENCIL_INSTANCE(double, 4, 32, 128, 2)    \
+  PF5D_STENCIL_INSTANCE(double, 8, 16, 128, 3)    \
+  PF5D_STENCIL_INSTANCE(float, 8, 32, 256, 3)     \
+  PF5D_STENCIL_INSTANCE(float, 8, 32, 256, 2)     \
+  PF5D_STENCIL_INSTANCE(float, 16, 32, 256, 2)    \
+  PF5D_STENCIL_INSTANCE(float, 8, 64, 256, 2)     \
+  PF5D_STENCIL_INSTANCE(float, 4, 64, 256, 3)     \
+  PF5D_STENCIL_INSTANCE(float, 8, 16, 128, 4)     \
+  PF5D_POTENTIALS_INSTANCE(double, 256, 2)        \
+  PF5D_POTENTIALS_INSTANCE(float, 256, 3)         \
+  PF5D_POTENTIALS_INSTANCE(float, 256, 4)         \
+  PF5D_FLUX_INSTANCE(double, 256, false)          \
+  PF5D_FLUX_INSTANCE(double, 256, true)           \
+  PF5D_FLUX_INSTANCE(float, 256, false)           \
+  PF5D_FLUX_INSTANCE(float, 256, true)
+
+}  // namespace pf5d

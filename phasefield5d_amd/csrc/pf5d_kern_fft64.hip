@@ -1,0 +1,7 @@
+// Instances of the batched FFT kernel in double (see pf5d_instances.h)
+#define PF5D_INST template
+#include "pf5d_instances.h"
+
+namespace pf5d {
+PF5D_FFT64_INSTANCES
+}  // namespace pf5d

@@ -32,21 +32,6 @@ struct SplitSmem {
   static constexpr int BOXMAX = sizeof(T) == 4 ? 256 : 128;
 };
 
-// XCDs (dies with their own L2) of the chip; workgroups are handed to them
-// round robin, workgroup b to XCD b % NXCD
-constexpr int NXCD = 8;
-
-// This block's contiguous item range [ib, ie).  Consecutive ranges go to
-// blocks on the same XCD, so the neighbouring planes a block reads were
-// brought into that XCD's L2 by the blocks working next to it.
-PF5D_DEV void block_range(int nitems, int& ib, int& ie) {
-  const int G = gridDim.x, b = blockIdx.x;
-  const int L = (G % NXCD == 0) ? (b % NXCD) * (G / NXCD) + b / NXCD : b;
-  const int per = (nitems + G - 1) / G;
-  ib = min(nitems, L * per);
-  ie = min(nitems, ib + per);
-}
-
 // cell index of the tile thread, and whether it has a cell
 template <typename T>
 PF5D_DEV bool tile_cell(const SimDev<T>& sd, const K1Item& item, int tid, float rtz, int& gx, int& gy, int& gz) {
@@ -164,7 +149,7 @@ PF5D_DEV void face_ab(const CellV<T>& a, const CellV<T>& b, T inv_dx, T out[4]) 
   face_flux<T>(a.X.v, b.X.v, a.M, b.M, g, out);
 }
 
-template <typename T, int NT>
+template <typename T, int NT, bool EAGER>
 __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st,
                                                 T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
                                                 int nitems) {
@@ -202,37 +187,44 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
     if (!tile_cell<T>(sd, item, tid, 1.0f / (float)item.tz, gx, gy, gz)) continue;
     const long c = ((long)gx * n1 + gy) * n2 + gz;
     const T inv_dx = sd.inv_dx;
+    const long sx = (long)n1 * n2;
+    const long ixu = hx ? c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx) : c;
+    const long ixd = hx ? c + (gx > 0 ? -sx : (long)(n0 - 1) * sx) : c;
+    const long iyu = hy ? c + (long)(wrap(gy + 1, n1) - gy) * n2 : c;
+    const long iyd = hy ? c + (long)(wrap(gy - 1, n1) - gy) * n2 : c;
+    const long izu = c + (wrap(gz + 1, n2) - gz), izd = c + (wrap(gz - 1, n2) - gz);
     const CellV<T> me = load_cell<T>(sd, c);
-    T acc[4], fp[4], fm[4];
+    // EAGER: all seven cells are requested before any face is evaluated (one
+    // memory round trip, more registers); absent axes read the cell itself
+    CellV<T> pre[6];
+    if constexpr (EAGER) {
+      pre[0] = load_cell<T>(sd, ixu);
+      pre[1] = load_cell<T>(sd, ixd);
+      pre[2] = load_cell<T>(sd, iyu);
+      pre[3] = load_cell<T>(sd, iyd);
+      pre[4] = load_cell<T>(sd, izu);
+      pre[5] = load_cell<T>(sd, izd);
+    }
+    T acc[4];
     bool first = true;
     // divergence in the reference order: x, y, z, each (F+ - F-) added in turn
-    if (hx) {
-      const long s = (long)n1 * n2;
-      const CellV<T> up = load_cell<T>(sd, c + (gx + 1 < n0 ? s : s - (long)n0 * s));
-      const CellV<T> dn = load_cell<T>(sd, c + (gx > 0 ? -s : (long)(n0 - 1) * s));
-      face_ab<T>(me, up, inv_dx, fp);
-      face_ab<T>(dn, me, inv_dx, fm);
-#pragma unroll
-      for (int a = 0; a < 4; ++a) acc[a] = fp[a] - fm[a];
-      first = false;
-    }
-    if (hy) {
-      const CellV<T> up = load_cell<T>(sd, c + (long)(wrap(gy + 1, n1) - gy) * n2);
-      const CellV<T> dn = load_cell<T>(sd, c + (long)(wrap(gy - 1, n1) - gy) * n2);
-      face_ab<T>(me, up, inv_dx, fp);
-      face_ab<T>(dn, me, inv_dx, fm);
+    auto axis = [&](int a0, long iu, long id) {
+      T fp[4], fm[4];
+      if constexpr (EAGER) {
+        face_ab<T>(me, pre[a0], inv_dx, fp);
+        face_ab<T>(pre[a0 + 1], me, inv_dx, fm);
+      } else {
+        const CellV<T> up = load_cell<T>(sd, iu), dn = load_cell<T>(sd, id);
+        face_ab<T>(me, up, inv_dx, fp);
+        face_ab<T>(dn, me, inv_dx, fm);
+      }
 #pragma unroll
       for (int a = 0; a < 4; ++a) acc[a] = first ? (fp[a] - fm[a]) : ((acc[a] + fp[a]) - fm[a]);
       first = false;
-    }
-    {
-      const CellV<T> up = load_cell<T>(sd, c + (wrap(gz + 1, n2) - gz));
-      const CellV<T> dn = load_cell<T>(sd, c + (wrap(gz - 1, n2) - gz));
-      face_ab<T>(me, up, inv_dx, fp);
-      face_ab<T>(dn, me, inv_dx, fm);
-#pragma unroll
-      for (int a = 0; a < 4; ++a) acc[a] = first ? (fp[a] - fm[a]) : ((acc[a] + fp[a]) - fm[a]);
-    }
+    };
+    if (hx) axis(0, ixu, ixd);
+    if (hy) axis(2, iyu, iyd);
+    axis(4, izu, izd);
     V4<T> o;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {

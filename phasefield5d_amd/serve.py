@@ -60,7 +60,11 @@ def footprint_bytes(pb, precision):
     b = n * 4 * w * (3 if precision == "fp32" else 2)     # X, dX (+ remainder in FP32)
     b += n * 9 * w                                         # mu, Mob of the two-pass stencil
     if pb.Bhat is not None:
-        nh = int(np.prod(pb.Bhat.shape))
+        kshape = list(pb.Bhat.shape)
+        if len(kshape) > 1:                                # spectrum rows padded to 128-byte lines
+            per = 128 // (2 * w)
+            kshape[-1] = -(-kshape[-1] // per) * per
+        nh = int(np.prod(kshape))
         b += n * 2 * w + nh * 3 * w                        # psi, s, spectrum (complex), Bhat
     return b
 
