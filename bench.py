@@ -12,8 +12,11 @@ Reported:
   fp64_plain   the per-function drop-in FP64 path (one device call per reference
                function, host dt from abs_max each step)
   error        max / RMS |X_fp32 - X_fp64| after the timed steps (same init)
-  roofline     fused stencil kernel K1: algorithmic bytes (36 B/cell FP32:
-               X 16 + psi 4 read, dX/dt 16 written) / its HIP-event duration
+  roofline     the dominant kernel of the FP32 step: with the two-pass stencil
+               (the autotuner's usual pick) the potentials pass, algorithmic
+               bytes 56 B/cell (X 16 + psi 4 read; mu 16 + Mob 20 written); with
+               the single-pass stencil K1, 36 B/cell (X 16 + psi 4 read, dX/dt 16
+               written); achieved = bytes / the kernel's HIP-event duration
   cpu_baseline C/OpenMP FP64 restatement of the reference loop + scipy.fft
 """
 import argparse
@@ -164,7 +167,14 @@ def main():
     v32 = ncell * args.steps / el32
     k1_ms = e32.time_kernel("k1", reps=20)
     fft_ms = e32.time_kernel("fft", reps=10)
+    k2_ms = e32.time_kernel("k2prime", reps=10)
+    dt_ms = e32.time_kernel("dt", reps=10)
     info = e32.launch_info()
+    split = e32.variants()[info["variant"]][0] == 0 if "variant" in info else False
+    kp_ms = kf_ms = None
+    if split:
+        kp_ms = e32.time_kernel("k1_potentials", reps=20)
+        kf_ms = e32.time_kernel("k1_flux", reps=20)
     log(f"fp32: {v32:.4g} cell-updates/s, {1e3 * el32 / args.steps:.3f} ms/step; K1 {k1_ms:.3f} ms, FFT {fft_ms:.3f} ms",
         info)
     st32 = e32.state(s32)
@@ -207,14 +217,18 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
+    if split:
+        kname, kfile, bpc, kms = ("k_potentials<float> (interpolation from the LDS node box + kappa lap + psi -> mu, Mob)",
+                                  "k_potentials_3d128_fp32_traffic.json", 56, kp_ms)
+    else:
+        kname, kfile, bpc, kms = ("k_stencil<float> (fused interpolation + kappa lap + mu + face flux + divergence)",
+                                  "k_stencil_3d128_fp32_traffic.json", 36, k1_ms)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_stencil_3d128_fp32_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", kfile)
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    k1_bytes = 36 * ncell
-    achieved = k1_bytes / (k1_ms * 1e-3) / 1e9
+    achieved = bpc * ncell / (kms * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": v32, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el32 / args.steps, "higher_is_better": True,
@@ -223,16 +237,16 @@ def main():
                                "elasticity on, synthetic regular-solution table",
                    "cells": ncell, "shape": list(pb.shape), "precision": "fp32", "elastic": True},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stencil<float> (fused interpolation + kappa lap + mu + face flux + divergence)",
-                     "bytes_per_cell": 36, "avg_ms": k1_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname,
+                     "bytes_per_cell": bpc, "avg_ms": kms},
         "fp64_fused": {"value": v64, "ms_per_step": 1e3 * el64 / args.steps if el64 else None, "k1_ms": k1_64},
         "fp64_plain": {"value": vplain, "steps": args.plain_steps},
         "ratio_fp32_vs_plain_fp64": v32 / vplain if vplain else None,
         "error_vs_fp64": {"max": float(diff.max()), "rms": float(np.sqrt((diff ** 2).mean())),
                           "steps": args.warmup + args.steps},
         "mass_drift_fp32": drift32.tolist(),
-        "stage_ms": {"k1": k1_ms, "fft": fft_ms},
+        "stage_ms": {"k1": k1_ms, "k1_potentials": kp_ms, "k1_flux": kf_ms, "fft": fft_ms, "k2": k2_ms,
+                     "dt": dt_ms},
         "launch": info,
     }
     if not args.fp32_only and not args.no_mix:

@@ -135,8 +135,10 @@ void* pf5d_engine_field_ptr(pf5d_engine* e, int slot, int which);
 void* pf5d_engine_stream(pf5d_engine* e);
 /* number of active slots and bytes of device memory held */
 int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes);
-/* last K1/K2/FFT launch configuration, for profiling reports: [k1 items, k2 items, fft items, k1 grid, k1 lds] */
-int pf5d_engine_launch_info(pf5d_engine* e, long out5[5]);
+/* launch configuration of the current work lists, for profiling reports: [K1 items, K2 items,
+ * FFT items, K1 grid, K1 LDS bytes, stencil variant, x chunk, FFT LDS budget bytes, FFT block
+ * size, flux-pass grid] */
+int pf5d_engine_launch_info(pf5d_engine* e, long out[10]);
 
 /* average device time of one launch of a stage (HIP events on the engine stream):
  * which = 0 stencil stage K1 (either form), 1 the FFT passes, 2 dt controller, 3 update

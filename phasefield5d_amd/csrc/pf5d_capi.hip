@@ -1088,11 +1088,17 @@ struct Engine : EngineBase {
     return (double)ms / reps;
   }
   void launch_info(long* out) override {
+    if (dirty) rebuild();
     out[0] = nk1;
     out[1] = nk2;
     out[2] = fft.offs[5];
     out[3] = k1_grid;
     out[4] = (long)variants[variant].lds;
+    out[5] = variant;
+    out[6] = xchunk;
+    out[7] = (long)fft.budget;
+    out[8] = fft.nt;
+    out[9] = kf_grid;
   }
 };
 
@@ -1405,8 +1411,8 @@ int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes) {
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms) {
   return guarded([&] { *avg_ms = e->impl->time_kernel(which, reps); });
 }
-int pf5d_engine_launch_info(pf5d_engine* e, long out5[5]) {
-  return guarded([&] { e->impl->launch_info(out5); });
+int pf5d_engine_launch_info(pf5d_engine* e, long out[10]) {
+  return guarded([&] { e->impl->launch_info(out); });
 }
 
 }  // extern "C"

@@ -156,6 +156,7 @@ class Engine:
         return ms.value
 
     def launch_info(self):
-        out = (C.c_long * 5)()
+        out = (C.c_long * 10)()
         _lib.check(_lib.lib().pf5d_engine_launch_info(self.h, out))
-        return dict(zip(("k1_items", "k2_items", "fft_items", "k1_grid", "k1_lds"), list(out)))
+        return dict(zip(("k1_items", "k2_items", "fft_items", "k1_grid", "k1_lds", "variant", "cx", "fft_budget",
+                         "fft_nt", "kf_grid"), list(out)))
