@@ -542,9 +542,9 @@ static K1Variant<T> make_variant() {
 }
 // MINW: register budget of the potentials pass; EAGER: the flux pass loads all
 // seven cells up front
-template <typename T, int NT, int MINW, bool EAGER>
+template <typename T, int NT, int MINW, bool EAGER, int BZ = 64>
 static K1Variant<T> make_split_variant() {
-  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, EAGER>};
+  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, EAGER>};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -554,7 +554,7 @@ template <>
 std::vector<K1Variant<double>> k1_variants<double>() {
   return {make_variant<double, 8, 32, 256, 2>(), make_variant<double, 4, 32, 128, 2>(),
           make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, false>(),
-          make_split_variant<double, 256, 2, true>()};
+          make_split_variant<double, 256, 2, true>(), make_split_variant<double, 256, 2, true, 32>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
@@ -562,11 +562,11 @@ std::vector<K1Variant<float>> k1_variants<float>() {
           make_variant<float, 16, 32, 256, 2>(), make_variant<float, 8, 64, 256, 2>(),
           make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>(),
           make_split_variant<float, 256, 3, false>(), make_split_variant<float, 256, 4, false>(),
-          make_split_variant<float, 256, 4, true>()};
+          make_split_variant<float, 256, 4, true>(), make_split_variant<float, 256, 4, true, 32>()};
 }
 
 constexpr int K2_NT = 256;
-constexpr long K2_CHUNK = 8192;
+constexpr long K2_CHUNK = 2048;  // cells per K2 item: 1024 items (16 waves per CU) at 128^3
 constexpr int DT_NT = 256;
 
 template <typename T>

@@ -512,6 +512,7 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
     T ref[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) ref[a] = sd.X0[a];
+#pragma unroll 2
     for (long c = item.c0 + threadIdx.x; c < item.c1; c += NT) {
       V4<T> x = ld4g<T>(sd.X + 4 * c);
       double xd[4];  // the state in FP64 (FP32 mode: X + Xlo)
