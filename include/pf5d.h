@@ -65,6 +65,29 @@ int pf5d_kappa_subtract(int prec, void* lap, void* mu, long mstride, long ncell,
 int pf5d_divergence_from_mu(int prec, const void* X, const void* mob, long bstride, const void* mu, long mstride,
                             int dim, const long* shape, double inv_dx, void* out, void* stream);
 
+/* calculate_gradients_pm(array, cell_size, grad_p, grad_m) -- solver/operators.py:88-158.
+ * A: rows of 4 components with row stride astride (e.g. the mu view of the interpolated data);
+ * grad_p / grad_m: (dim, ncell, 4), forward (A(c+e_d) - A(c))/h and backward (A(c) - A(c-e_d))/h. */
+int pf5d_gradients_pm(int prec, const void* A, long astride, int dim, const long* shape, double h, void* grad_p,
+                      void* grad_m, void* stream);
+
+/* get_flux_function(dim)(X, Mob, grad_mu_p, grad_mu_m, fluxes_p, fluxes_m) -- solver/fluxes.py:6-125.
+ * mob rows of stride bstride; gradients and fluxes (dim, ncell, 4).  Used for the save-step
+ * total_flux diagnostic (examples/simulate.py:302-312, 323). */
+int pf5d_fluxes_pm(int prec, const void* X, const void* mob, long bstride, int dim, const long* shape,
+                   const void* grad_p, const void* grad_m, void* fluxes_p, void* fluxes_m, void* stream);
+
+/* Save-step field statistics of an (ncell, ncomp <= 8) array with row stride astride, in FP64:
+ * out[0..n) sums (compensated), [n..2n) minima, [2n..3n) maxima, [3n..4n) population variances
+ * about the FP64 mean, [4n] maximum row sum -- the reductions of examples/simulate.py:299-346
+ * (net, mass, comp_min/max, comp_var, comp_fe_min).  Blocks until out is on the host. */
+int pf5d_field_stats(int prec, const void* A, long ncell, long astride, int ncomp, double* out, void* stream);
+
+/* Per-component FP64 dot products out[k] = sum_c A[c,k] B[c,k] of two contiguous (ncell, ncomp <= 8)
+ * arrays: the sum of X . kappa lap X in the save-step gradient energy (examples/simulate.py:336-345).
+ * Blocks until out is on the host. */
+int pf5d_field_dot(int prec, const void* A, const void* B, long ncell, int ncomp, double* out, void* stream);
+
 /* abs_max -- solver/operators.py:161-175.  Blocks until the value is on the host. */
 int pf5d_abs_max(int prec, const void* a, long n, double* host_result, void* stream);
 

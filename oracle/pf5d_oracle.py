@@ -161,6 +161,25 @@ def divergence_from_mu(X, Mob, mu, inv_dx):
     return acc * inv_dx
 
 
+def gradients_pm(A, h):
+    """calculate_gradients_pm (solver/operators.py:88-158): (dim, *spatial, n) forward
+    (A(c+e_d) - A(c)) / h and backward (A(c) - A(c-e_d)) / h differences."""
+    dim = A.ndim - 1
+    inv_dx = 1.0 / h
+    gp = np.stack([(_nbr(A, d, 1) - A) * inv_dx for d in range(dim)])
+    gm = np.stack([(A - _nbr(A, d, -1)) * inv_dx for d in range(dim)])
+    return gp, gm
+
+
+def fluxes_pm(X, Mob, gp, gm):
+    """get_flux_function(dim) (solver/fluxes.py:6-125): face fluxes towards the +/- neighbour
+    along every axis, (dim, *spatial, 4) each."""
+    dim = X.ndim - 1
+    fp = np.stack([face_flux(X, _nbr(X, d, 1), Mob, _nbr(Mob, d, 1), gp[d]) for d in range(dim)])
+    fm = np.stack([face_flux(X, _nbr(X, d, -1), Mob, _nbr(Mob, d, -1), gm[d]) for d in range(dim)])
+    return fp, fm
+
+
 # ---------------------------------------------------------------------------
 # rank-1 elastic term (solver/elastic.py:21-29, 246-263)
 # ---------------------------------------------------------------------------

@@ -36,6 +36,10 @@ _SIGS = {
     "pf5d_kappa_subtract": (i32, [i32, vp, vp, i64, i64, P_f64, vp]),
     "pf5d_divergence_from_mu": (i32, [i32, vp, vp, i64, vp, i64, i32, P_i64, f64, vp, vp]),
     "pf5d_abs_max": (i32, [i32, vp, i64, P_f64, vp]),
+    "pf5d_gradients_pm": (i32, [i32, vp, i64, i32, P_i64, f64, vp, vp, vp]),
+    "pf5d_fluxes_pm": (i32, [i32, vp, vp, i64, i32, P_i64, vp, vp, vp, vp, vp]),
+    "pf5d_field_stats": (i32, [i32, vp, i64, i64, i32, P_f64, vp]),
+    "pf5d_field_dot": (i32, [i32, vp, vp, i64, i32, P_f64, vp]),
     "pf5d_update": (i32, [i32, vp, vp, i64, f64, vp]),
     "pf5d_clamp_simplex": (i32, [i32, vp, i64, vp]),
     "pf5d_bounds_stats": (i32, [i32, vp, i64, P_f64, vp]),

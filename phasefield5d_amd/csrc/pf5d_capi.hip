@@ -1224,6 +1224,120 @@ int pf5d_divergence_from_mu(int prec, const void* X, const void* mob, long bstri
   });
 }
 
+int pf5d_gradients_pm(int prec, const void* A, long astride, int dim, const long* shape, double h, void* grad_p,
+                      void* grad_m, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    require(astride >= 4 && h > 0, "gradients: need row stride >= 4 and h > 0");
+    Shape sh = make_shape(dim, shape);
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_gradients_pm<double>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream),
+                         (const double*)A, astride, sh, 1.0 / h, (double*)grad_p, (double*)grad_m);
+    else
+      hipLaunchKernelGGL(k_gradients_pm<float>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream),
+                         (const float*)A, astride, sh, (float)(1.0 / h), (float*)grad_p, (float*)grad_m);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_fluxes_pm(int prec, const void* X, const void* mob, long bstride, int dim, const long* shape,
+                   const void* grad_p, const void* grad_m, void* fluxes_p, void* fluxes_m, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    require(bstride >= 5, "fluxes: mobility row stride must be >= 5");
+    Shape sh = make_shape(dim, shape);
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_fluxes_pm<double>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream),
+                         (const double*)X, (const double*)mob, bstride, sh, (const double*)grad_p,
+                         (const double*)grad_m, (double*)fluxes_p, (double*)fluxes_m);
+    else
+      hipLaunchKernelGGL(k_fluxes_pm<float>, dim3(nblocks(sh.ncell, 256)), dim3(256), 0, S(stream),
+                         (const float*)X, (const float*)mob, bstride, sh, (const float*)grad_p,
+                         (const float*)grad_m, (float*)fluxes_p, (float*)fluxes_m);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int pf5d_field_stats(int prec, const void* A, long ncell, long astride, int ncomp, double* out, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    require(ncell > 0 && ncomp >= 1 && ncomp <= 8 && astride >= ncomp && out, "field_stats: bad arguments");
+    const unsigned grid = (unsigned)std::min<long>(nblocks(ncell, 256), 4L * devinfo().cus);
+    double* d = (double*)scratch().get((size_t)(64 + grid * 24) * sizeof(double));
+    double *acc = d, *mean = d + 40, *part = d + 64, *qpart = part + (size_t)grid * 16;
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(64), 0, st, acc);
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_stats<double>, dim3(grid), dim3(256), 0, st, (const double*)A, ncell, astride, ncomp,
+                         part, acc);
+    else
+      hipLaunchKernelGGL(k_stats<float>, dim3(grid), dim3(256), 0, st, (const float*)A, ncell, astride, ncomp,
+                         part, acc);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hp((size_t)grid * 16), ha(33);
+    HIPCHK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ha.data(), acc, ha.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double m[8];
+    for (int k = 0; k < ncomp; ++k) {
+      double sum = 0, c = 0;
+      for (unsigned b = 0; b < grid; ++b) {
+        for (double v : {hp[b * 16 + k], hp[b * 16 + 8 + k]}) {
+          const double t = sum + v;
+          c += (std::fabs(sum) >= std::fabs(v)) ? ((sum - t) + v) : ((v - t) + sum);
+          sum = t;
+        }
+      }
+      out[k] = sum + c;
+      m[k] = out[k] / (double)ncell;
+      out[ncomp + k] = ha[16 + k];
+      out[2 * ncomp + k] = ha[24 + k];
+    }
+    out[4 * ncomp] = ha[32];
+    HIPCHK(hipMemcpyAsync(mean, m, ncomp * sizeof(double), hipMemcpyHostToDevice, st));
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_sqdev<double>, dim3(grid), dim3(256), 0, st, (const double*)A, ncell, astride, ncomp,
+                         mean, qpart);
+    else
+      hipLaunchKernelGGL(k_sqdev<float>, dim3(grid), dim3(256), 0, st, (const float*)A, ncell, astride, ncomp,
+                         mean, qpart);
+    HIPCHK(hipGetLastError());
+    std::vector<double> hq((size_t)grid * 8);
+    HIPCHK(hipMemcpyAsync(hq.data(), qpart, hq.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < ncomp; ++k) {
+      double q = 0;
+      for (unsigned b = 0; b < grid; ++b) q += hq[b * 8 + k];
+      out[3 * ncomp + k] = q / (double)ncell;
+    }
+  });
+}
+
+int pf5d_field_dot(int prec, const void* A, const void* B, long ncell, int ncomp, double* out, void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    require(ncell > 0 && ncomp >= 1 && ncomp <= 8 && out, "field_dot: bad arguments");
+    const unsigned grid = (unsigned)std::min<long>(nblocks(ncell, 256), 4L * devinfo().cus);
+    double* part = (double*)scratch().get((size_t)grid * 8 * sizeof(double));
+    hipStream_t st = S(stream);
+    if (prec == PF5D_F64)
+      hipLaunchKernelGGL(k_dot<double>, dim3(grid), dim3(256), 0, st, (const double*)A, (const double*)B, ncell,
+                         ncomp, part);
+    else
+      hipLaunchKernelGGL(k_dot<float>, dim3(grid), dim3(256), 0, st, (const float*)A, (const float*)B, ncell, ncomp,
+                         part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h((size_t)grid * 8);
+    HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < ncomp; ++k) {
+      double q = 0;
+      for (unsigned b = 0; b < grid; ++b) q += h[b * 8 + k];
+      out[k] = q;
+    }
+  });
+}
+
 int pf5d_abs_max(int prec, const void* a, long n, double* host_result, void* stream) {
   return guarded([&] {
     check_prec(prec);

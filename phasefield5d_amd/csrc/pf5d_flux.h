@@ -183,6 +183,196 @@ __global__ void k_div_from_mu(const T* __restrict__ X, const T* __restrict__ mob
   st4<T>(out + 4 * c, o);
 }
 
+// calculate_gradients_pm (operators.py:88-158): per axis d, forward
+// (A(c+e_d) - A(c)) / h and backward (A(c) - A(c-e_d)) / h differences of a
+// 4-component field with row stride astride; out (dim, ncell, 4)
+template <typename T>
+__global__ void k_gradients_pm(const T* __restrict__ A, long astride, Shape sh, T inv_dx, T* __restrict__ gp,
+                               T* __restrict__ gm) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= sh.ncell) return;
+  for (int a = 0; a < sh.dim; ++a) {
+    const long ip = nbr_index(sh, c, a, 1), im = nbr_index(sh, c, a, -1);
+    V4<T> p, m;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const T center = A[c * astride + s];
+      p.v[s] = (A[ip * astride + s] - center) * inv_dx;
+      m.v[s] = (center - A[im * astride + s]) * inv_dx;
+    }
+    st4<T>(gp + 4 * ((long)a * sh.ncell + c), p);
+    st4<T>(gm + 4 * ((long)a * sh.ncell + c), m);
+  }
+}
+
+// get_flux_function(dim) (fluxes.py:6-125): the flux through both faces of
+// every cell along every axis from the gradients above; out (dim, ncell, 4)
+template <typename T>
+__global__ void k_fluxes_pm(const T* __restrict__ X, const T* __restrict__ mob, long bstride, Shape sh,
+                            const T* __restrict__ gp, const T* __restrict__ gm, T* __restrict__ fp,
+                            T* __restrict__ fm) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= sh.ncell) return;
+  T Xc[4], Mc[5];
+  const V4<T> xv = ld4<T>(X + 4 * c);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) Xc[s] = xv.v[s];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) Mc[m] = mob[c * bstride + m];
+  for (int a = 0; a < sh.dim; ++a) {
+    for (int st = 1; st >= -1; st -= 2) {
+      const long n = nbr_index(sh, c, a, st);
+      const long o = 4 * ((long)a * sh.ncell + c);
+      T Xn[4], Mn[5], g[4], f[4];
+      const V4<T> xn = ld4<T>(X + 4 * n), gv = ld4<T>((st > 0 ? gp : gm) + o);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        Xn[s] = xn.v[s];
+        g[s] = gv.v[s];
+      }
+#pragma unroll
+      for (int m = 0; m < 5; ++m) Mn[m] = mob[n * bstride + m];
+      face_flux<T>(Xc, Xn, Mc, Mn, g, f);
+      V4<T> fv;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) fv.v[s] = f[s];
+      st4<T>((st > 0 ? fp : fm) + o, fv);
+    }
+  }
+}
+
+// Field statistics of an (ncell, ncomp <= 8) array with row stride astride,
+// in FP64: per component a Neumaier sum, min and max, and the max of the row
+// sums.  acc[0..ncomp) sums, [8..16) compensations, [16..24) min, [24..32)
+// max, [32] max row sum; acc is prepared by the caller (k_stats_init).
+static __global__ void k_stats_init(double* acc) {
+  const int t = threadIdx.x;
+  if (t < 16) acc[t] = 0.0;
+  else if (t < 24) acc[t] = 1e300;
+  else if (t < 33) acc[t] = -1e300;
+}
+static PF5D_DEV void atomic_min_double(double* a, double v) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(a);
+  unsigned long long old = *p;
+  while (v < __longlong_as_double((long long)old)) {
+    const unsigned long long prev = atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+static PF5D_DEV void atomic_max_double(double* a, double v) {
+  unsigned long long* p = reinterpret_cast<unsigned long long*>(a);
+  unsigned long long old = *p;
+  while (v > __longlong_as_double((long long)old)) {
+    const unsigned long long prev = atomicCAS(p, old, (unsigned long long)__double_as_longlong(v));
+    if (prev == old) break;
+    old = prev;
+  }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_stats(const T* __restrict__ A, long ncell, long astride, int ncomp,
+                                               double* __restrict__ part, double* __restrict__ acc) {
+  __shared__ double red[4][33];
+  double s[8], cmp[8], mn[8], mx[8], rmax = -1e300;
+  for (int k = 0; k < 8; ++k) {
+    s[k] = cmp[k] = 0.0;
+    mn[k] = 1e300;
+    mx[k] = -1e300;
+  }
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x) {
+    double rs = 0.0;
+    for (int k = 0; k < ncomp; ++k) {
+      const double v = (double)A[c * astride + k];
+      const double t = s[k] + v;
+      cmp[k] += (fabs(s[k]) >= fabs(v)) ? ((s[k] - t) + v) : ((v - t) + s[k]);
+      s[k] = t;
+      mn[k] = fmin(mn[k], v);
+      mx[k] = fmax(mx[k], v);
+      rs += v;
+    }
+    rmax = fmax(rmax, rs);
+  }
+  // block reduction: plain sums of the per-thread Neumaier sums (the compensation
+  // terms are carried separately), then one partial per block
+  for (int k = 0; k < ncomp; ++k) {
+    double a = s[k], b = cmp[k], lo = mn[k], hi = mx[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+      lo = fmin(lo, __shfl_xor(lo, off, 64));
+      hi = fmax(hi, __shfl_xor(hi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[threadIdx.x >> 6][k] = a;
+      red[threadIdx.x >> 6][8 + k] = b;
+      red[threadIdx.x >> 6][16 + k] = lo;
+      red[threadIdx.x >> 6][24 + k] = hi;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][32] = rmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < ncomp; ++k) {
+      double a = 0, b = 0, lo = 1e300, hi = -1e300;
+      for (int w = 0; w < 4; ++w) {
+        a += red[w][k];
+        b += red[w][8 + k];
+        lo = fmin(lo, red[w][16 + k]);
+        hi = fmax(hi, red[w][24 + k]);
+      }
+      part[blockIdx.x * 16 + k] = a;
+      part[blockIdx.x * 16 + 8 + k] = b;
+      atomic_min_double(&acc[16 + k], lo);
+      atomic_max_double(&acc[24 + k], hi);
+    }
+    double r = red[0][32];
+    for (int w = 1; w < 4; ++w) r = fmax(r, red[w][32]);
+    atomic_max_double(&acc[32], r);
+  }
+}
+// second pass: sum of squared deviations from the given per-component means
+template <typename T>
+__global__ void __launch_bounds__(256) k_sqdev(const T* __restrict__ A, long ncell, long astride, int ncomp,
+                                               const double* __restrict__ mean, double* __restrict__ part) {
+  __shared__ double red[4][8];
+  double q[8];
+  for (int k = 0; k < 8; ++k) q[k] = 0.0;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x)
+    for (int k = 0; k < ncomp; ++k) {
+      const double d = (double)A[c * astride + k] - mean[k];
+      q[k] += d * d;
+    }
+  for (int k = 0; k < ncomp; ++k) {
+    double a = q[k];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < ncomp; ++k) part[blockIdx.x * 8 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+}
+
+// per-component FP64 dot products sum_c A[c,k] B[c,k] of two (ncell, ncomp)
+// arrays (contiguous rows), one partial per block
+template <typename T>
+__global__ void __launch_bounds__(256) k_dot(const T* __restrict__ A, const T* __restrict__ B, long ncell, int ncomp,
+                                             double* __restrict__ part) {
+  __shared__ double red[4][8];
+  double q[8];
+  for (int k = 0; k < 8; ++k) q[k] = 0.0;
+  for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += (long)gridDim.x * blockDim.x)
+    for (int k = 0; k < ncomp; ++k) q[k] += (double)A[c * ncomp + k] * (double)B[c * ncomp + k];
+  for (int k = 0; k < ncomp; ++k) {
+    double a = q[k];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][k] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < ncomp; ++k) part[blockIdx.x * 8 + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+}
+
 template <typename T>
 __global__ void k_fill(T* p, long n, T v) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -35,27 +35,47 @@ class PlainStepper:
         self.xmax = 0.0
         self.stopped = False
 
-    def step(self, oob_interval=100):
-        """One reference step; returns False (and stops) on a hard bounds violation."""
-        if self.stopped:
-            return False
+    def potentials(self):
+        """mu and Mob of the current state into P (simulate.py:242-263); returns (mu, Mob) views."""
         pb, cfg = self.pb, self.pb.cfg
-        if oob_interval and self.timestep % oob_interval == 0 and out_of_bounds(self.X):
-            self.stopped = True
-            return False
         interpolator_nb(self.X, pb.data_grid, cfg.resolution, pb.tree, pb.calphad_values, out=self.P)
         mu = self.P[..., :4]
         mob = self.P[..., 4:]
         calculate_laplacian(self.X, pb.dx, self.L)
         kappa_subtract(self.L, mu, self.kappa)
         self.elastic_update(mu, self.X)
+        return mu, mob
+
+    def step(self, oob_interval=100, save=None):
+        """One reference step; returns False (and stops) on a hard bounds violation.
+
+        ``save(timestep, t_after) -> (save_data, save_snapshot)`` is asked once dt is
+        known (simulate.py:290-297); on a data save the step's diagnostics are left in
+        ``self.last_diag`` (pre-update ones from this step's state, post-update ones after
+        the clamp, as simulate.py:299-346 computes them)."""
+        if self.stopped:
+            return False
+        pb, cfg = self.pb, self.pb.cfg
+        if oob_interval and self.timestep % oob_interval == 0 and out_of_bounds(self.X):
+            self.stopped = True
+            return False
+        mu, mob = self.potentials()
         self.div(self.X, mob, mu, 1.0 / pb.dx, self.dX)
         self.xmax = float(abs_max(self.dX))
         self.dt = apply_adaptive_time(self.dt, self.xmax, pb.ctld, pb.dt_cfl,
                                       composition_change_lower_limit=cfg.lower_limit,
                                       composition_change_upper_limit=cfg.upper_limit)
+        self.last_diag = None
+        sd = False
+        if save is not None:
+            sd, _ = save(self.timestep, self.t + self.dt)
+        if sd:
+            from . import diagnostics
+            self.last_diag = diagnostics.pre_update(self.X, mu, mob, self.dX, pb.dx)
         self.t += self.dt
         update_composition(self.X, self.dX, self.dt)
         clamp_simplex(self.X)
+        if sd:
+            self.last_diag.update(diagnostics.post_update(self.X, pb.dx, self.kappa))
         self.timestep += 1
         return True

@@ -14,5 +14,5 @@ from .elastic import (  # noqa: F401
     make_elastic_updater, build_khachaturyan_kernel, calculate_linear_elastic_coupling_matrix, get_elastic_matrix,
 )
 from .interpolation import build_4d_grid, build_calphad_kdtree, interpolator_nb  # noqa: F401
-from .operators import calculate_laplacian, abs_max, clamp_simplex  # noqa: F401
-from .fluxes import get_divergence_from_mu_function  # noqa: F401
+from .operators import calculate_laplacian, calculate_gradients_pm, abs_max, clamp_simplex  # noqa: F401
+from .fluxes import get_divergence_from_mu_function, get_flux_function  # noqa: F401

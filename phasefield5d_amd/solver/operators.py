@@ -1,7 +1,8 @@
 """Finite-difference operators of the explicit step (phasefield5d/solver/operators.py).
 
 Device implementations with the reference signatures: periodic Laplacian,
-serial-semantics abs-max (NaN ignored), in-place simplex clamp.
+forward/backward gradients, serial-semantics abs-max (NaN ignored), in-place
+simplex clamp, and the FP64 field statistics of the save steps.
 """
 import ctypes as C
 
@@ -26,6 +27,59 @@ def calculate_laplacian(array, cell_size, out=None):
     _lib.check(_lib.lib().pf5d_laplacian(prec, C.c_void_p(Xd.data_ptr()), dim, shape, float(cell_size),
                                         O.ptr(), _lib.stream_handle()))
     return O.finish()
+
+
+def calculate_gradients_pm(array, cell_size, grad_p=None, grad_m=None):
+    """Forward (+) and backward (-) differences along each axis (operators.py:88-158).
+
+    ``array`` is (*spatial, 4), possibly a row-strided view (the mu columns of the
+    interpolated data); returns (grad_p, grad_m) of shape (dim, *spatial, 4).
+    """
+    from ..device import row_stride
+    dim = array.ndim - 1
+    if dim not in (1, 2, 3):
+        raise NotImplementedError(f"Gradients not implemented for spatial_dims={dim}")
+    prec = prec_of(array)
+    A = Staged(array)
+    gshape = (dim,) + tuple(A.dev.shape)
+    if grad_p is None:
+        grad_p = np.empty(gshape, dtype=np.asarray(array[:0]).dtype) if A.host is not None else A.dev.new_empty(gshape)
+    if grad_m is None:
+        grad_m = np.empty_like(grad_p) if A.host is not None else A.dev.new_empty(gshape)
+    P = Staged(grad_p, copy_in=False)
+    M = Staged(grad_m, copy_in=False)
+    shape = _lib.larr(A.dev.shape[:-1])
+    _lib.check(_lib.lib().pf5d_gradients_pm(prec, A.ptr(), row_stride(A.dev, 4), dim, shape, float(cell_size),
+                                           P.ptr(), M.ptr(), _lib.stream_handle()))
+    return P.finish(), M.finish()
+
+
+def field_stats(array):
+    """FP64 statistics over the spatial axes of a (*spatial, n) field (n <= 8), for the
+    save-step diagnostics of examples/simulate.py:299-346: dict of per-component
+    ``sum``, ``min``, ``max``, ``var`` (population, about the mean) and ``max_row_sum``."""
+    from ..device import row_stride
+    prec = prec_of(array)
+    A = Staged(array)
+    n = A.dev.shape[-1]
+    ncell = A.dev.numel() // n if A.dev.is_contiguous() else int(np.prod(A.dev.shape[:-1]))
+    out = (C.c_double * (4 * n + 1))()
+    _lib.check(_lib.lib().pf5d_field_stats(prec, A.ptr(), ncell, row_stride(A.dev, n), n, out,
+                                          _lib.stream_handle()))
+    v = np.frombuffer(out, dtype=np.float64).copy()
+    return {"sum": v[:n], "min": v[n:2 * n], "max": v[2 * n:3 * n], "var": v[3 * n:4 * n], "max_row_sum": v[4 * n]}
+
+
+def field_dot(a, b):
+    """Per-component FP64 sums of a * b over the spatial axes of two (*spatial, n) fields."""
+    prec = prec_of(a)
+    A, B = Staged(a), Staged(b)
+    Ad, Bd = A.dev.contiguous(), B.dev.contiguous()
+    n = Ad.shape[-1]
+    out = (C.c_double * n)()
+    _lib.check(_lib.lib().pf5d_field_dot(prec, C.c_void_p(Ad.data_ptr()), C.c_void_p(Bd.data_ptr()),
+                                        Ad.numel() // n, n, out, _lib.stream_handle()))
+    return np.frombuffer(out, dtype=np.float64).copy()
 
 
 def abs_max(arr):

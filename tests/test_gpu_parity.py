@@ -230,6 +230,61 @@ def test_retiling_midrun_keeps_trajectory(dev, precision):
     assert np.allclose(st["mass"], pb.X_init.reshape(-1, 4).sum(axis=0), rtol=1e-12)
 
 
+@pytest.mark.parametrize("shape", [(40,), (12, 10), (8, 6, 10)])
+def test_gradients_and_fluxes_bitwise(dev, shape):
+    """calculate_gradients_pm and get_flux_function equal the oracle bit for bit (FP64),
+    for NumPy arrays and for the strided mu / Mob views of device data."""
+    from phasefield5d_amd.solver.fluxes import get_flux_function
+    from phasefield5d_amd.solver.operators import calculate_gradients_pm
+    rng = np.random.default_rng(21)
+    dim, h = len(shape), 2e-9
+    X = rng.dirichlet(np.ones(5) * 4, size=shape)[..., 1:]
+    P = np.concatenate([rng.normal(size=shape + (4,)) * 1e3, rng.uniform(0.5, 2.0, shape + (5,)) * 1e-24], axis=-1)
+    mu, mob = P[..., :4], P[..., 4:]
+    ogp, ogm = O.gradients_pm(mu, h)
+    ofp, ofm = O.fluxes_pm(X, mob, ogp, ogm)
+    gp, gm = calculate_gradients_pm(mu, h)
+    assert np.array_equal(gp, ogp) and np.array_equal(gm, ogm)
+    fp, fm = np.empty_like(gp), np.empty_like(gm)
+    get_flux_function(dim)(X, mob, gp, gm, fp, fm)
+    assert np.array_equal(fp, ofp) and np.array_equal(fm, ofm)
+    # device tensors with row-strided views (stride 9)
+    Pd, Xd = torch.from_numpy(P).cuda(), torch.from_numpy(X).cuda()
+    gpd, gmd = calculate_gradients_pm(Pd[..., :4], h)
+    fpd, fmd = torch.empty_like(gpd), torch.empty_like(gmd)
+    get_flux_function(dim)(Xd, Pd[..., 4:], gpd, gmd, fpd, fmd)
+    assert np.array_equal(fpd.cpu().numpy(), ofp) and np.array_equal(fmd.cpu().numpy(), ofm)
+
+
+def test_save_step_diagnostics_vs_oracle(dev):
+    """diagnostics.pre_update / post_update against NumPy sums of the oracle's fields
+    (examples/simulate.py:299-346), FP64 sums in a different order: 1e-12 relative."""
+    from phasefield5d_amd import diagnostics
+    rng = np.random.default_rng(22)
+    shape, h = (12, 10, 8), 1e-9
+    kappa = np.array([1.0, 2.0, 0.5, 1.5]) * 5e-16
+    X = rng.dirichlet(np.ones(5) * 4, size=shape)[..., 1:]
+    P = np.concatenate([rng.normal(size=shape + (4,)) * 1e3, rng.uniform(0.5, 2.0, shape + (5,)) * 1e-24], axis=-1)
+    dX = rng.normal(size=shape + (4,)) * 1e-3
+    mu, mob = P[..., :4], P[..., 4:]
+    gp, gm = O.gradients_pm(mu, h)
+    fp, fm = O.fluxes_pm(X, mob, gp, gm)
+    Pd = torch.from_numpy(P).cuda()
+    pre = diagnostics.pre_update(torch.from_numpy(X).cuda(), Pd[..., :4], Pd[..., 4:], torch.from_numpy(dX).cuda(), h)
+    axes = (0, 1, 2)
+    assert np.allclose(pre["net"], dX.sum(axis=axes), rtol=1e-12, atol=1e-12 * np.abs(dX).sum())
+    assert np.array_equal(pre["max_per_comp"], np.abs(dX).max(axis=axes))
+    tot = (fp - fm).sum(axis=(0, 1, 2, 3))
+    assert np.abs(pre["total_flux"] - tot).max() <= 1e-12 * np.abs(fp).sum()
+    post = diagnostics.post_update(torch.from_numpy(X).cuda(), h, kappa)
+    assert np.allclose(post["mass"], X.sum(axis=axes), rtol=1e-13)
+    assert np.array_equal(post["comp_min"], X.min(axis=axes)) and np.array_equal(post["comp_max"], X.max(axis=axes))
+    assert np.allclose(post["comp_var"], X.var(axis=axes), rtol=1e-10)
+    assert post["comp_fe_min"] == 1.0 - X.sum(axis=-1).max()
+    fg = -0.5 * h ** 3 * float(np.sum(X * (O.laplacian(X, h) * kappa)))
+    assert abs(post["f_grad"] - fg) <= 1e-10 * abs(fg)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_stencil_forms_bitwise(dev, precision):
     """Every stencil tiling, single-pass or two-pass, gives the same bits: rows

@@ -155,6 +155,49 @@ def test_divergence_uniform_composition_fourier_mu(shape):
     assert np.abs(d.reshape(-1, 4).sum(axis=0)).max() < 1e-12 * np.abs(d).max() * d.size ** 0.5
 
 
+@pytest.mark.parametrize("shape", [(20,), (12, 10), (6, 8, 10)])
+def test_gradients_pm_closed_form_and_shift(shape):
+    """Forward difference of cos(theta x) is -2 sin(theta/2) sin(theta (x + 1/2)) / h;
+    the backward difference at c is the forward one at c - e_d (same subtraction)."""
+    h = 2e-9
+    dim = len(shape)
+    grids = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    m = [[1, 2, 1, 3][a % 4] for a in range(4)]
+    A = np.stack([np.cos(2 * np.pi * m[a] * grids[a % dim] / shape[a % dim]) for a in range(4)], axis=-1)
+    gp, gm = O.gradients_pm(A, h)
+    assert gp.shape == (dim,) + shape + (4,)
+    for d in range(dim):
+        for a in range(4):
+            expect = np.zeros(shape)
+            if a % dim == d:
+                th = 2 * np.pi * m[a] / shape[d]
+                expect = -2 * np.sin(th / 2) * np.sin(th * (grids[d] + 0.5)) / h
+            assert np.allclose(gp[d][..., a], expect, atol=1e-6 / h)
+        assert np.array_equal(gm[d], np.roll(gp[d], 1, axis=d))
+
+
+@pytest.mark.parametrize("shape", [(16,), (8, 12), (6, 6, 8)])
+def test_fluxes_pm_symmetry_and_divergence(shape):
+    """fm at c is fp at c - e_d bit for bit (the face is shared), the fluxes assemble
+    the fused divergence exactly (fluxes.py:393-395), and their total telescopes to 0."""
+    rng = np.random.default_rng(11)
+    dim = len(shape)
+    h = 1e-9
+    X = rng.dirichlet(np.ones(5) * 4, size=shape)[..., 1:]
+    Mob = rng.uniform(0.5, 2.0, shape + (5,)) * 1e-24
+    mu = rng.normal(size=shape + (4,)) * 1e3
+    gp, gm = O.gradients_pm(mu, h)
+    fp, fm = O.fluxes_pm(X, Mob, gp, gm)
+    for d in range(dim):
+        assert np.array_equal(fm[d], np.roll(fp[d], 1, axis=d))
+    acc = None
+    for d in range(dim):
+        acc = (fp[d] - fm[d]) if acc is None else (acc + fp[d]) - fm[d]
+    assert np.array_equal(acc * (1.0 / h), O.divergence_from_mu(X, Mob, mu, 1.0 / h))
+    tot = (fp - fm).reshape(-1, 4).sum(axis=0)
+    assert np.abs(tot).max() <= 1e-12 * np.abs(fp).max() * fp.size ** 0.5
+
+
 def test_elastic_psi_closed_forms():
     from phasefield5d_amd.solver.elastic import build_khachaturyan_kernel
     from phasefield5d_amd.solver.system import build_fourier_grid_1d_along_direction, build_fourier_grid_3d
