@@ -672,6 +672,7 @@ struct Engine : EngineBase {
   void add(const pf5d_sim_desc* d, const double* Xh, const double* bhat, const pf5d_table* tab, int* slot) override {
     require(d->dim >= 1 && d->dim <= 3, "dim must be 1, 2 or 3", PF5D_ENOTIMPL);
     require(tab && tab->prec == (sizeof(T) == 8 ? PF5D_F64 : PF5D_F32), "table precision does not match engine");
+    require(tab->num >= BOX4, "the engine needs a table with at least 4 nodes per axis");
     long ncell = 1;
     for (int a = 0; a < d->dim; ++a) {
       require(d->shape[a] >= 4, "every axis needs at least 4 cells");

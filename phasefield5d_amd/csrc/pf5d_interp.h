@@ -232,6 +232,38 @@ PF5D_DEV void interp9_staged(const TableView<T>& tv, const T* box, const NodeBox
   }
 }
 
+// A fixed box of 4^4 table nodes with lower node lo[] (rows of NPAD, node
+// (q0,q1,q2,q3) at row ((q0*4 + q1)*4 + q2)*4 + q3), staged in LDS: points whose
+// lower cube corner lies in its first three nodes on every axis interpolate from
+// it, the others from the table in HBM (same arithmetic either way).  The strides
+// are compile-time constants, so the corner offsets fold into the LDS reads.
+constexpr int BOX4 = 4;
+template <typename T>
+PF5D_DEV void interp9_box4(const TableView<T>& tv, const T* box, const int lo[4], const T X[4], T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  const T s = ((X[0] + X[1]) + X[2]) + X[3];
+  if (!(s <= tv.thresh)) {
+    interp_nearest<T>(tv, X, P);
+    return;
+  }
+  const int num = tv.num;
+  int i0, i1, i2, i3;
+  T t0, t1, t2, t3;
+  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
+  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
+  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
+  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
+  const int d0 = i0 - lo[0], d1 = i1 - lo[1], d2 = i2 - lo[2], d3 = i3 - lo[3];
+  if ((unsigned)d0 <= BOX4 - 2 && (unsigned)d1 <= BOX4 - 2 && (unsigned)d2 <= BOX4 - 2 &&
+      (unsigned)d3 <= BOX4 - 2) {
+    const T* base = box + (((d0 * BOX4 + d1) * BOX4 + d2) * BOX4 + d3) * NP;
+    interp_core<T, true>(base, BOX4 * NP, BOX4 * BOX4 * NP, BOX4 * BOX4 * BOX4 * NP, t0, t1, t2, t3, P);
+  } else {
+    const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
+    interp_core<T, false>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);
+  }
+}
+
 // floor index of the lower cube corner on one axis (same rule as axis_index)
 template <typename T>
 PF5D_DEV int floor_index(T u, int num) {

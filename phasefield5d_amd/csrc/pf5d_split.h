@@ -1,8 +1,10 @@
 // Two-pass form of the stencil stage: the same arithmetic as k_stencil,
 // operation for operation, so both forms give the same bits.
 //
-//   KP  k_potentials : per cell: interpolation (LDS node box) + kappa lap(X)
-//                      + elastic psi -> mu (4 values) and Mob (5 values) in HBM
+//   KP  k_potentials : per cell: interpolation + kappa lap(X) + elastic psi
+//                      -> mu (4 values) and Mob (5 values) in HBM; the table
+//                      nodes around the tile's first cell (4^4) are staged in
+//                      LDS and serve every point whose cube lies among them
 //   KF  k_fluxdiv    : per cell: the face fluxes to its neighbours from
 //                      (X, mu, Mob) -> dX/dt, max |dX/dt| per simulation
 //
@@ -27,11 +29,6 @@
 
 namespace pf5d {
 
-template <typename T>
-struct SplitSmem {
-  static constexpr int BOXMAX = sizeof(T) == 4 ? 256 : 128;
-};
-
 // cell index of the tile thread, and whether it has a cell
 template <typename T>
 PF5D_DEV bool tile_cell(const SimDev<T>& sd, const K1Item& item, int tid, float rtz, int& gx, int& gy, int& gz) {
@@ -48,10 +45,11 @@ template <typename T, int NT, int MINW>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st, const K1Item* __restrict__ items,
              int nitems) {
-  constexpr int BOXMAX = SplitSmem<T>::BOXMAX;
   constexpr int NP = TablePad<T>::value;
-  __shared__ __align__(16) T BOX[BOXMAX * NP];
-  __shared__ int BRED[(NT / 64) * 8 + 8];
+  constexpr int NBOX = BOX4 * BOX4 * BOX4 * BOX4;
+  using VT = typename Vec16<T>::type;
+  constexpr int NCOL = NP / Vec16<T>::n;  // 16-byte columns per table row
+  __shared__ __align__(16) T BOX[NBOX * NP];
   int ib, ie;
   block_range(nitems, ib, ie);
   for (int it = ib; it < ie; ++it) {
@@ -69,14 +67,25 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
     const long c = ((long)gx * n1 + gy) * n2 + gz;
     V4<T> xc{};
     if (valid) xc = ld4g<T>(sd.X + 4 * c);
-    int v[8];
+    // node box around the tile's first cell (the anchor): lower nodes one below its
+    // lower cube corners, so corners within +-1 of the anchor's are covered
+    const int num = sd.tab.num;
+    int lo[4];
+    {
+      const V4<T> xa = ld4g<T>(sd.X + 4 * (((long)item.xa * n1 + item.y0) * n2 + item.z0));
 #pragma unroll
-    for (int a = 0; a < 8; ++a) v[a] = 0x7fffffff;
-    if (valid && ((xc.v[0] + xc.v[1]) + xc.v[2]) + xc.v[3] <= sd.tab.thresh) node_box_point<T>(sd.tab, xc.v, v);
-    const NodeBox nb = stage_node_box<T, NT, BOXMAX>(sd.tab, v, BOX, BRED, tid);
+      for (int a = 0; a < 4; ++a) lo[a] = min(max(floor_index<T>(xa.v[a] * sd.tab.inv_step, num) - 1, 0), num - BOX4);
+    }
+    for (int e = tid; e < NBOX * NCOL; e += NT) {
+      const int node = e / NCOL, col = e - node * NCOL;
+      const long row = ((((long)(lo[0] + (node >> 6)) * num + lo[1] + ((node >> 4) & 3)) * num + lo[2] +
+                        ((node >> 2) & 3)) * num + lo[3] + (node & 3));
+      reinterpret_cast<VT*>(BOX)[e] = gld<VT>(sd.tab.data + row * NP + col * Vec16<T>::n);
+    }
+    __syncthreads();
     if (valid) {
       T P[9];
-      interp9_staged<T>(sd.tab, BOX, nb, xc.v, P);
+      interp9_box4<T>(sd.tab, BOX, lo, xc.v, P);
       // neighbour sums in the reference order: x, then y, then z (each up + down)
       T acc[4];
       bool first = true;
@@ -117,7 +126,7 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
       st4g<T>(sd.mb + 4 * c, mb);
       gst<T>(sd.mb1 + c, P[8]);
     }
-    __syncthreads();  // BOX / BRED are reused by the next item
+    __syncthreads();  // BOX is reused by the next item
   }
 }
 
