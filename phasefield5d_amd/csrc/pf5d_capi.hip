@@ -540,11 +540,12 @@ template <typename T, int BY, int BZ, int NT, int MINW>
 static K1Variant<T> make_variant() {
   return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT, MINW>, nullptr, nullptr};
 }
-// MINW: register budget of the potentials pass; EAGER: the flux pass loads all
-// seven cells up front
-template <typename T, int NT, int MINW, bool EAGER, int BZ = 64>
+// MINW: register budget of the potentials pass; LOADS: how the flux pass
+// fetches the neighbour cells (0 per axis, 1 all up front, 2 up front with the
+// z neighbours from adjacent lanes); BZ: row length of a tile
+template <typename T, int NT, int MINW, int LOADS, int BZ = 64>
 static K1Variant<T> make_split_variant() {
-  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, EAGER>};
+  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, LOADS>};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -553,16 +554,18 @@ static std::vector<K1Variant<T>> k1_variants();
 template <>
 std::vector<K1Variant<double>> k1_variants<double>() {
   return {make_variant<double, 8, 32, 256, 2>(), make_variant<double, 4, 32, 128, 2>(),
-          make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, false>(),
-          make_split_variant<double, 256, 2, true>(), make_split_variant<double, 256, 2, true, 32>()};
+          make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, 0>(),
+          make_split_variant<double, 256, 2, 1>(), make_split_variant<double, 256, 2, 1, 32>(),
+          make_split_variant<double, 256, 2, 2>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
   return {make_variant<float, 8, 32, 256, 3>(), make_variant<float, 8, 32, 256, 2>(),
           make_variant<float, 16, 32, 256, 2>(), make_variant<float, 8, 64, 256, 2>(),
           make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>(),
-          make_split_variant<float, 256, 3, false>(), make_split_variant<float, 256, 4, false>(),
-          make_split_variant<float, 256, 4, true>(), make_split_variant<float, 256, 4, true, 32>()};
+          make_split_variant<float, 256, 3, 0>(), make_split_variant<float, 256, 4, 0>(),
+          make_split_variant<float, 256, 4, 1>(), make_split_variant<float, 256, 4, 1, 32>(),
+          make_split_variant<float, 256, 4, 2>()};
 }
 
 constexpr int K2_NT = 256;

@@ -26,8 +26,8 @@ namespace pf5d {
 #define PF5D_POTENTIALS_INSTANCE(T, NT, MINW)                                                              \
   PF5D_INST __global__ void k_potentials<T, NT, MINW>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
                                                       const K1Item* __restrict__, int);
-#define PF5D_FLUX_INSTANCE(T, NT, EAGER)                                                                       \
-  PF5D_INST __global__ void k_fluxdiv<T, NT, EAGER>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
+#define PF5D_FLUX_INSTANCE(T, NT, LOADS)                                                                       \
+  PF5D_INST __global__ void k_fluxdiv<T, NT, LOADS>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
                                                     T* __restrict__, const K1Item* __restrict__, int);
 
 #define PF5D_FFT32_INSTANCES      \
@@ -55,9 +55,11 @@ namespace pf5d {
   PF5D_POTENTIALS_INSTANCE(double, 256, 2)        \
   PF5D_POTENTIALS_INSTANCE(float, 256, 3)         \
   PF5D_POTENTIALS_INSTANCE(float, 256, 4)         \
-  PF5D_FLUX_INSTANCE(double, 256, false)          \
-  PF5D_FLUX_INSTANCE(double, 256, true)           \
-  PF5D_FLUX_INSTANCE(float, 256, false)           \
-  PF5D_FLUX_INSTANCE(float, 256, true)
+  PF5D_FLUX_INSTANCE(double, 256, 0)              \
+  PF5D_FLUX_INSTANCE(double, 256, 1)              \
+  PF5D_FLUX_INSTANCE(double, 256, 2)              \
+  PF5D_FLUX_INSTANCE(float, 256, 0)               \
+  PF5D_FLUX_INSTANCE(float, 256, 1)               \
+  PF5D_FLUX_INSTANCE(float, 256, 2)
 
 }  // namespace pf5d

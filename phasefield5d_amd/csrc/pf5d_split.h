@@ -158,7 +158,25 @@ PF5D_DEV void face_ab(const CellV<T>& a, const CellV<T>& b, T inv_dx, T out[4]) 
   face_flux<T>(a.X.v, b.X.v, a.M, b.M, g, out);
 }
 
-template <typename T, int NT, bool EAGER>
+// the X, mu and Mob of lane (lane - 1) (dir = -1) or (lane + 1) (dir = +1) of the wavefront
+template <typename T>
+PF5D_DEV CellV<T> shfl_cell(const CellV<T>& v, int dir) {
+  CellV<T> r;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    r.X.v[a] = dir < 0 ? __shfl_up(v.X.v[a], 1, 64) : __shfl_down(v.X.v[a], 1, 64);
+    r.mu.v[a] = dir < 0 ? __shfl_up(v.mu.v[a], 1, 64) : __shfl_down(v.mu.v[a], 1, 64);
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) r.M[m] = dir < 0 ? __shfl_up(v.M[m], 1, 64) : __shfl_down(v.M[m], 1, 64);
+  return r;
+}
+
+// LOADS: 0 = the two neighbours of an axis loaded when the axis is evaluated;
+// 1 = all seven cells requested before any face is evaluated (one memory round
+// trip, more registers); 2 = as 1, but the z neighbours come from the adjacent
+// lanes of the wavefront (consecutive z), loaded only at row and wavefront ends
+template <typename T, int NT, int LOADS>
 __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st,
                                                 T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
                                                 int nitems) {
@@ -193,7 +211,14 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
     const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
     const bool hx = sd.dim == 3, hy = sd.dim >= 2;
     int gx, gy, gz;
-    if (!tile_cell<T>(sd, item, tid, 1.0f / (float)item.tz, gx, gy, gz)) continue;
+    const bool valid = tile_cell<T>(sd, item, tid, 1.0f / (float)item.tz, gx, gy, gz);
+    if constexpr (LOADS < 2) {
+      if (!valid) continue;
+    } else if (!valid) {
+      // lanes past the tile's rows take part in the lane exchange on the tile's first cell
+      gy = item.y0;
+      gz = item.z0;
+    }
     const long c = ((long)gx * n1 + gy) * n2 + gz;
     const T inv_dx = sd.inv_dx;
     const long sx = (long)n1 * n2;
@@ -203,23 +228,30 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
     const long iyd = hy ? c + (long)(wrap(gy - 1, n1) - gy) * n2 : c;
     const long izu = c + (wrap(gz + 1, n2) - gz), izd = c + (wrap(gz - 1, n2) - gz);
     const CellV<T> me = load_cell<T>(sd, c);
-    // EAGER: all seven cells are requested before any face is evaluated (one
-    // memory round trip, more registers); absent axes read the cell itself
+    // absent axes read the cell itself (not used)
     CellV<T> pre[6];
-    if constexpr (EAGER) {
+    if constexpr (LOADS >= 1) {
       pre[0] = load_cell<T>(sd, ixu);
       pre[1] = load_cell<T>(sd, ixd);
       pre[2] = load_cell<T>(sd, iyu);
       pre[3] = load_cell<T>(sd, iyd);
+    }
+    if constexpr (LOADS == 1) {
       pre[4] = load_cell<T>(sd, izu);
       pre[5] = load_cell<T>(sd, izd);
+    } else if constexpr (LOADS == 2) {
+      const int lane = tid & 63, k = gz - item.z0;
+      pre[4] = shfl_cell<T>(me, +1);
+      pre[5] = shfl_cell<T>(me, -1);
+      if (!(k < item.tz - 1 && lane < 63)) pre[4] = load_cell<T>(sd, izu);
+      if (!(k > 0 && lane > 0)) pre[5] = load_cell<T>(sd, izd);
     }
     T acc[4];
     bool first = true;
     // divergence in the reference order: x, y, z, each (F+ - F-) added in turn
     auto axis = [&](int a0, long iu, long id) {
       T fp[4], fm[4];
-      if constexpr (EAGER) {
+      if constexpr (LOADS >= 1) {
         face_ab<T>(me, pre[a0], inv_dx, fp);
         face_ab<T>(pre[a0 + 1], me, inv_dx, fm);
       } else {
@@ -236,11 +268,12 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
     axis(4, izu, izd);
     V4<T> o;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      o.v[a] = acc[a] * inv_dx;
-      tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
+    for (int a = 0; a < 4; ++a) o.v[a] = acc[a] * inv_dx;
+    if (valid) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
+      st4g<T>(sd.dX + 4 * c, o);
     }
-    st4g<T>(sd.dX + 4 * c, o);
   }
   if (cur >= 0) flush(cur);
 }
