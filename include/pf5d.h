@@ -135,8 +135,11 @@ int pf5d_engine_destroy(pf5d_engine* e);
 int pf5d_engine_add(pf5d_engine* e, const pf5d_sim_desc* d, const double* X_host, const double* bhat_host,
                     const pf5d_table* tab, int* slot);
 int pf5d_engine_remove(pf5d_engine* e, int slot);
-/* K1 tile: variant index (pf5d_engine_num_variants), x chunk (3D), per dimension */
-int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* by_bz_nt /* 3*n ints, may be NULL */);
+/* K1 tile: variant index (pf5d_engine_num_variants), x chunk (3D), per dimension.
+ * info (5*n ints, may be NULL), per variant: by, bz, nt (by == 0: two-pass form),
+ * then the flux pass of the two-pass form: cells per lane (0: per-plane tiles,
+ * > 0: marching along axis 0 in x chunks) and its block size */
+int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* info);
 int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx);
 /* FFT passes: LDS bytes per work item in KiB (sets how many lines one block
  * transforms) and threads per block (128 or 256) */

@@ -54,9 +54,11 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
         return ent
     has3d = any(len(pb.shape) == 3 for pb, _ in engine.slots.values())
     results = []
-    for v, (by, bz, nt) in enumerate(engine.variants()):
-        # by == 0 is the two-pass form (no x marching, so no chunk length)
-        for cx in (chunks if has3d and by else chunks[:1]):
+    for v, (by, bz, nt, fcpl, fnt) in enumerate(engine.variants()):
+        # the x-chunk length applies to the single-pass form (by > 0) and to the
+        # marching flux pass of the two-pass form (fcpl > 0)
+        marching = by > 0 or fcpl > 0
+        for cx in (chunks if has3d and marching else chunks[:1]):
             try:
                 engine.set_tiling(v, cx)
                 ms = engine.time_kernel("k1", reps=reps)

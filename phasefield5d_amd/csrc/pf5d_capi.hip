@@ -533,6 +533,10 @@ struct K1Variant {
   void (*fn)(const SimDev<T>*, SimState*, T*, const K1Item*, int);
   void (*fpot)(const SimDev<T>*, const SimState*, const K1Item*, int);
   void (*fflux)(const SimDev<T>*, const SimState*, T*, const K1Item*, int);
+  // flux pass of the two-pass form: fcpl == 0 runs on the potentials pass's
+  // per-plane tiles; fcpl > 0 marches along axis 0 over tiles of rows of
+  // 64*fcpl cells, fnt/64 rows per block
+  int fcpl = 0, fnt = 0;
   bool split() const { return by == 0; }
 };
 
@@ -545,7 +549,12 @@ static K1Variant<T> make_variant() {
 // z neighbours from adjacent lanes); BZ: row length of a tile
 template <typename T, int NT, int MINW, int LOADS, int BZ = 64>
 static K1Variant<T> make_split_variant() {
-  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, LOADS>};
+  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, LOADS>, 0, NT};
+}
+// two-pass form with the marching flux pass (FNT threads, CPL cells per lane)
+template <typename T, int NT, int MINW, int FNT, int CPL>
+static K1Variant<T> make_march_variant() {
+  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxmarch<T, FNT, CPL>, CPL, FNT};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -556,7 +565,7 @@ std::vector<K1Variant<double>> k1_variants<double>() {
   return {make_variant<double, 8, 32, 256, 2>(), make_variant<double, 4, 32, 128, 2>(),
           make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, 0>(),
           make_split_variant<double, 256, 2, 1>(), make_split_variant<double, 256, 2, 1, 32>(),
-          make_split_variant<double, 256, 2, 2>()};
+          make_split_variant<double, 256, 2, 2>(), make_march_variant<double, 256, 2, 256, 1>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
@@ -565,7 +574,8 @@ std::vector<K1Variant<float>> k1_variants<float>() {
           make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>(),
           make_split_variant<float, 256, 3, 0>(), make_split_variant<float, 256, 4, 0>(),
           make_split_variant<float, 256, 4, 1>(), make_split_variant<float, 256, 4, 1, 32>(),
-          make_split_variant<float, 256, 4, 2>()};
+          make_split_variant<float, 256, 4, 2>(), make_march_variant<float, 256, 4, 256, 1>(),
+          make_march_variant<float, 256, 4, 256, 2>(), make_march_variant<float, 256, 4, 512, 2>()};
 }
 
 constexpr int K2_NT = 256;
@@ -595,10 +605,11 @@ struct Engine : EngineBase {
   SimState* d_state = nullptr;
   T* d_xmax = nullptr;
   K1Item* d_k1 = nullptr;
+  K1Item* d_kf = nullptr;  // tiles of the marching flux pass
   K2Item* d_k2 = nullptr;
   double* d_part = nullptr;
   int* d_prange = nullptr;
-  int nk1 = 0, nk2 = 0, cap = 0;
+  int nk1 = 0, nkf = 0, nk2 = 0, cap = 0;
   FftBatch<T> fft;
   bool dirty = true;
   hipGraphExec_t g1 = nullptr, gN = nullptr;
@@ -631,6 +642,7 @@ struct Engine : EngineBase {
     dfree(d_state);
     dfree(d_xmax);
     dfree(d_k1);
+    dfree(d_kf);
     dfree(d_k2);
     dfree(d_part);
     dfree(d_prange);
@@ -816,7 +828,7 @@ struct Engine : EngineBase {
     drop_graphs();
     const K1Variant<T>& V = variants[variant];
     std::vector<SimDev<T>> devs(cap);
-    std::vector<K1Item> k1;
+    std::vector<K1Item> k1, kf;
     std::vector<K2Item> k2;
     std::vector<int> prange(2 * cap, 0);
     fft.clear();
@@ -839,6 +851,19 @@ struct Engine : EngineBase {
             const int tz = std::min(tzmax, v.n2 - z0), ry = V.nt / tz;
             for (int y0 = 0; y0 < v.n1; y0 += ry)
               for (int x = xa; x < xb; ++x) k1.push_back({i, x, x + 1, y0, z0, std::min(ry, v.n1 - y0), tz});
+          }
+        }
+        if (V.fcpl > 0) {
+          // marching flux tiles: fnt/64 rows of up to 64*fcpl cells, runs of
+          // xchunk planes, in the same slab order
+          const int rz = 64 * V.fcpl, ry = V.fnt / 64;
+          const int cxs = hx ? std::max(1, xchunk) : 1;
+          for (int sl = 0; sl < nsl; ++sl) {
+            const int xa = sl * v.n0 / nsl, xb = (sl + 1) * v.n0 / nsl;
+            for (int z0 = 0; z0 < v.n2; z0 += rz)
+              for (int y0 = 0; y0 < v.n1; y0 += ry)
+                for (int x = xa; x < xb; x += cxs)
+                  kf.push_back({i, x, std::min(x + cxs, xb), y0, z0, std::min(ry, v.n1 - y0), std::min(rz, v.n2 - z0)});
           }
         }
       }
@@ -868,13 +893,20 @@ struct Engine : EngineBase {
     HIPCHK(hipMemcpy(d_sims, devs.data(), cap * sizeof(SimDev<T>), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_prange, prange.data(), prange.size() * sizeof(int), hipMemcpyHostToDevice));
     dfree(d_k1);
+    dfree(d_kf);
     dfree(d_k2);
     dfree(d_part);
     d_k1 = nullptr;
+    d_kf = nullptr;
     d_k2 = nullptr;
     d_part = nullptr;
     nk1 = (int)k1.size();
+    nkf = (int)kf.size();
     nk2 = (int)k2.size();
+    if (nkf) {
+      d_kf = (K1Item*)dmalloc(nkf * sizeof(K1Item));
+      HIPCHK(hipMemcpy(d_kf, kf.data(), nkf * sizeof(K1Item), hipMemcpyHostToDevice));
+    }
     if (nk1) {
       d_k1 = (K1Item*)dmalloc(nk1 * sizeof(K1Item));
       HIPCHK(hipMemcpy(d_k1, k1.data(), nk1 * sizeof(K1Item), hipMemcpyHostToDevice));
@@ -891,8 +923,8 @@ struct Engine : EngineBase {
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fpot, V.nt, 0);
       k1_grid = grid(occ);
       occ = 1;
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fflux, V.nt, 0);
-      kf_grid = grid(occ);
+      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fflux, V.fnt, 0);
+      kf_grid = V.fcpl > 0 ? xcd_grid(std::min(nkf, devinfo().cus * std::max(1, occ))) : grid(occ);
     } else {
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
       occ = std::max(1, occ);
@@ -920,13 +952,19 @@ struct Engine : EngineBase {
                        d_part, d_prange, (int)sims.size(), advance);
     HIPCHK(hipGetLastError());
   }
+  void launch_flux(const K1Variant<T>& V) {
+    if (V.fcpl > 0)
+      hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.fnt), 0, stream, d_sims, d_state, d_xmax, d_kf, nkf);
+    else
+      hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+  }
   void launch_k1() {
     if (!nk1) return;
     const K1Variant<T>& V = variants[variant];
     if (V.split()) {
       hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
       HIPCHK(hipGetLastError());
-      hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+      launch_flux(V);
       HIPCHK(hipGetLastError());
       return;
     }
@@ -1053,9 +1091,11 @@ struct Engine : EngineBase {
   int num_variants(int* info) override {
     if (info)
       for (size_t i = 0; i < variants.size(); ++i) {
-        info[3 * i] = variants[i].by;
-        info[3 * i + 1] = variants[i].bz;
-        info[3 * i + 2] = variants[i].nt;
+        info[5 * i] = variants[i].by;
+        info[5 * i + 1] = variants[i].bz;
+        info[5 * i + 2] = variants[i].nt;
+        info[5 * i + 3] = variants[i].fcpl;
+        info[5 * i + 4] = variants[i].fnt;
       }
     return (int)variants.size();
   }
@@ -1078,7 +1118,7 @@ struct Engine : EngineBase {
       else if (which == 4)
         hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
       else
-        hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+        launch_flux(V);
     };
     one();
     HIPCHK(hipEventRecord(a, stream));

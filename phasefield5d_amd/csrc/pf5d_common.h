@@ -71,6 +71,13 @@ PF5D_DEV V sld(const void* p) {
   __builtin_memcpy(&v, &w, sizeof(V));
   return v;
 }
+template <typename V>
+PF5D_DEV void sst(void* p, const V& v) {
+  using W = typename Words<sizeof(V)>::type;
+  W w;
+  __builtin_memcpy(&w, &v, sizeof(V));
+  *(PF5D_LDS W*)p = w;
+}
 template <typename T>
 PF5D_DEV V4<T> ld4g(const T* p) { return gld<V4<T>>(p); }
 template <typename T>

@@ -26,6 +26,9 @@ namespace pf5d {
 #define PF5D_POTENTIALS_INSTANCE(T, NT, MINW)                                                              \
   PF5D_INST __global__ void k_potentials<T, NT, MINW>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
                                                       const K1Item* __restrict__, int);
+#define PF5D_FLUXM_INSTANCE(T, NT, CPL)                                                                          \
+  PF5D_INST __global__ void k_fluxmarch<T, NT, CPL>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
+                                                    T* __restrict__, const K1Item* __restrict__, int);
 #define PF5D_FLUX_INSTANCE(T, NT, LOADS)                                                                       \
   PF5D_INST __global__ void k_fluxdiv<T, NT, LOADS>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
                                                     T* __restrict__, const K1Item* __restrict__, int);
@@ -60,6 +63,10 @@ namespace pf5d {
   PF5D_FLUX_INSTANCE(double, 256, 2)              \
   PF5D_FLUX_INSTANCE(float, 256, 0)               \
   PF5D_FLUX_INSTANCE(float, 256, 1)               \
-  PF5D_FLUX_INSTANCE(float, 256, 2)
+  PF5D_FLUX_INSTANCE(float, 256, 2)               \
+  PF5D_FLUXM_INSTANCE(double, 256, 1)             \
+  PF5D_FLUXM_INSTANCE(float, 256, 1)              \
+  PF5D_FLUXM_INSTANCE(float, 256, 2)              \
+  PF5D_FLUXM_INSTANCE(float, 512, 2)
 
 }  // namespace pf5d

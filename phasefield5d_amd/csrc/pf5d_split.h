@@ -278,4 +278,305 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
   if (cur >= 0) flush(cur);
 }
 
+// 16 bytes of T, the unit of the LDS exchange below
+template <typename T>
+struct alignas(16) Q16 {
+  T v[16 / sizeof(T)];
+};
+
+// cell values of a lane, packed for the LDS exchange: X, mu, Mob of each cell
+template <typename T, int CPL, int QC>
+PF5D_DEV void put_cells(Q16<T> (*dst)[64], int lane, const CellV<T> (&c)[CPL]) {
+  constexpr int QN = 16 / sizeof(T);
+  T p[QC * QN];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      p[13 * j + a] = c[j].X.v[a];
+      p[13 * j + 4 + a] = c[j].mu.v[a];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) p[13 * j + 8 + m] = c[j].M[m];
+  }
+#pragma unroll
+  for (int i = 13 * CPL; i < QC * QN; ++i) p[i] = T(0);
+#pragma unroll
+  for (int q = 0; q < QC; ++q) {
+    Q16<T> v;
+#pragma unroll
+    for (int e = 0; e < QN; ++e) v.v[e] = p[q * QN + e];
+    sst<Q16<T>>(&dst[q][lane], v);
+  }
+}
+
+template <typename T, int CPL, int QC>
+PF5D_DEV void get_cells(const Q16<T> (*src)[64], int lane, CellV<T> (&c)[CPL]) {
+  constexpr int QN = 16 / sizeof(T);
+  T p[QC * QN];
+#pragma unroll
+  for (int q = 0; q < QC; ++q) {
+    const Q16<T> v = sld<Q16<T>>(&src[q][lane]);
+#pragma unroll
+    for (int e = 0; e < QN; ++e) p[q * QN + e] = v.v[e];
+  }
+#pragma unroll
+  for (int j = 0; j < CPL; ++j) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      c[j].X.v[a] = p[13 * j + a];
+      c[j].mu.v[a] = p[13 * j + 4 + a];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) c[j].M[m] = p[13 * j + 8 + m];
+  }
+}
+
+// X, mu, Mob of lane `src` of the wavefront
+template <typename T>
+PF5D_DEV CellV<T> shfl_cell_from(const CellV<T>& v, int src) {
+  CellV<T> r;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    r.X.v[a] = __shfl(v.X.v[a], src, 64);
+    r.mu.v[a] = __shfl(v.mu.v[a], src, 64);
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) r.M[m] = __shfl(v.M[m], src, 64);
+  return r;
+}
+
+// Flux pass marching along axis 0: every face is evaluated once.
+//
+// An item is a tile of rows (one per wavefront, at most NT/64) of up to 64*CPL
+// cells on axis 2, and a range [xa, xb) of planes on axis 0; lane i holds the
+// cells CPL*i .. CPL*i+CPL-1 of its wavefront's row.  Per plane:
+// - the x face below the plane is the x face above the previous plane (kept in
+//   registers), so one x face is evaluated per cell, plus one per cell for the
+//   range's first plane;
+// - the y face above a row uses the next row's cells, taken from LDS (the last
+//   row loads the row after the tile); the y face below a row is the face above
+//   the previous row, taken from LDS, and only the first row evaluates its own
+//   (or, when the tile spans the periodic axis, takes the last row's);
+// - the z face above a cell uses the next cell, taken from the same lane or by
+//   a lane shuffle; the z face below is the face above the previous cell, and
+//   only the row's first cell evaluates its own when the tile does not span the
+//   periodic row.
+// Faces are computed with the lower cell first, so a face evaluated once gives
+// the same bits as the reference's two evaluations (the face flux is symmetric),
+// and the divergence is summed in the reference order.
+template <typename T, int NT, int CPL>
+__global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st,
+                                                  T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
+                                                  int nitems) {
+  constexpr int NW = NT / 64;
+  constexpr int QN = 16 / sizeof(T);
+  constexpr int QC = (13 * CPL + QN - 1) / QN;  // 16-byte words per lane: cells
+  constexpr int QF = (4 * CPL + QN - 1) / QN;   // 16-byte words per lane: one face per cell
+  __shared__ Q16<T> SC[NW][QC][64];
+  __shared__ Q16<T> SF[NW][QF][64];
+  __shared__ T RED[NW];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  int ib, ie;
+  block_range(nitems, ib, ie);
+  T tmax = T(0);
+  int cur = -1;
+  auto flush = [&](int sim) {
+    const T m = wave_max<T>(tmax);
+    if (lane == 0) RED[w] = m;
+    __syncthreads();
+    if (tid == 0) {
+      T r = RED[0];
+      for (int v = 1; v < NW; ++v) r = (RED[v] > r) ? RED[v] : r;
+      atomic_max_nonneg(&xmax_acc[sim], r);
+    }
+    __syncthreads();
+    tmax = T(0);
+  };
+  for (int it = ib; it < ie; ++it) {
+    const K1Item item = items[it];
+    if (item.sim != cur) {
+      if (cur >= 0) flush(cur);
+      cur = item.sim;
+    }
+    if (!(st[item.sim].status & ST_ACTIVE)) continue;  // uniform per block
+    const SimDev<T>& sd = sims[item.sim];
+    const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
+    const bool hx = sd.dim == 3, hy = sd.dim >= 2;
+    const T inv_dx = sd.inv_dx;
+    // wavefronts past the tile's rows work on its first row and store nothing
+    const bool row_on = w < item.ty;
+    const int gy = item.y0 + (row_on ? w : 0);
+    const int tz = item.tz;
+    const bool zwrap = tz == n2, ywrap = item.ty == n1;
+    const int lend = (tz - 1) / CPL, jend = (tz - 1) - lend * CPL;  // lane and slot of the row's last cell
+    int kz[CPL];
+    bool kon[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      const int k = CPL * lane + j;
+      kon[j] = row_on && k < tz;
+      kz[j] = item.z0 + min(k, tz - 1);
+    }
+    auto row_base = [&](int x, int y) { return ((long)x * n1 + y) * n2; };
+    auto load_lane = [&](int x, int y, CellV<T>(&c)[CPL]) {
+      const long r = row_base(x, y);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) c[j] = load_cell<T>(sd, r + kz[j]);
+    };
+    CellV<T> me[CPL];
+    T fxd[CPL][4];
+    load_lane(item.xa, gy, me);
+    if (hx) {
+      CellV<T> pv[CPL];
+      load_lane(item.xa > 0 ? item.xa - 1 : n0 - 1, gy, pv);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) face_ab<T>(pv[j], me[j], inv_dx, fxd[j]);
+    }
+    for (int x = item.xa; x < item.xb; ++x) {
+      CellV<T> nx[CPL];
+      if (hx) load_lane(x + 1 < n0 ? x + 1 : 0, gy, nx);
+      T fyu[CPL][4], fyd[CPL][4];
+      if (hy) {
+        put_cells<T, CPL, QC>(SC[w], lane, me);
+        __syncthreads();
+        {
+          CellV<T> up[CPL];
+          if (w + 1 < item.ty)
+            get_cells<T, CPL, QC>(SC[w + 1], lane, up);
+          else if (ywrap)
+            get_cells<T, CPL, QC>(SC[0], lane, up);
+          else
+            load_lane(x, gy + 1 < n1 ? gy + 1 : 0, up);
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], up[j], inv_dx, fyu[j]);
+        }
+        {
+          T p[QF * QN];
+#pragma unroll
+          for (int j = 0; j < CPL; ++j)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) p[4 * j + a] = fyu[j][a];
+#pragma unroll
+          for (int i = 4 * CPL; i < QF * QN; ++i) p[i] = T(0);
+#pragma unroll
+          for (int q = 0; q < QF; ++q) {
+            Q16<T> v;
+#pragma unroll
+            for (int e = 0; e < QN; ++e) v.v[e] = p[q * QN + e];
+            sst<Q16<T>>(&SF[w][q][lane], v);
+          }
+        }
+        __syncthreads();
+        if (w > 0 || ywrap) {
+          const int src = (w > 0) ? w - 1 : item.ty - 1;
+          T p[QF * QN];
+#pragma unroll
+          for (int q = 0; q < QF; ++q) {
+            const Q16<T> v = sld<Q16<T>>(&SF[src][q][lane]);
+#pragma unroll
+            for (int e = 0; e < QN; ++e) p[q * QN + e] = v.v[e];
+          }
+#pragma unroll
+          for (int j = 0; j < CPL; ++j)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) fyd[j][a] = p[4 * j + a];
+        } else {
+          CellV<T> dn[CPL];
+          load_lane(x, gy > 0 ? gy - 1 : n1 - 1, dn);
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) face_ab<T>(dn[j], me[j], inv_dx, fyd[j]);
+        }
+      }
+      // z: the face above each cell, then the face below from the cell before
+      T fzu[CPL][4], fzd[CPL][4];
+      {
+        const CellV<T> nb = shfl_cell_from<T>(me[0], (lane + 1) & 63);  // the next lane's first cell
+        CellV<T> e;  // the cell above the row's last cell
+        if (zwrap) {
+          e = shfl_cell_from<T>(me[0], 0);
+        } else {
+          e = nb;
+          if (lane == lend) e = load_cell<T>(sd, row_base(x, gy) + (item.z0 + tz < n2 ? item.z0 + tz : 0));
+        }
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          const bool last = lane == lend && j == jend;
+          const CellV<T>& nxt = (j + 1 < CPL) ? me[j + 1 < CPL ? j + 1 : 0] : nb;
+          CellV<T> u;
+          u.X = last ? e.X : nxt.X;
+          u.mu = last ? e.mu : nxt.mu;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) u.M[m] = last ? e.M[m] : nxt.M[m];
+          face_ab<T>(me[j], u, inv_dx, fzu[j]);
+        }
+#pragma unroll
+        for (int j = 1; j < CPL; ++j)
+#pragma unroll
+          for (int a = 0; a < 4; ++a) fzd[j][a] = fzu[j - 1][a];
+        // the previous lane's last face; lane 0: the row's last face (periodic) or its own
+        T wl[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          T v = fzu[0][a];
+#pragma unroll
+          for (int j = 1; j < CPL; ++j) v = (jend == j) ? fzu[j][a] : v;
+          wl[a] = __shfl(v, lend, 64);
+          fzd[0][a] = __shfl(fzu[CPL - 1][a], (lane + 63) & 63, 64);
+        }
+        if (lane == 0) {
+          if (zwrap) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) fzd[0][a] = wl[a];
+          } else {
+            const CellV<T> dn = load_cell<T>(sd, row_base(x, gy) + (item.z0 > 0 ? item.z0 - 1 : n2 - 1));
+            face_ab<T>(dn, me[0], inv_dx, fzd[0]);
+          }
+        }
+      }
+      T fxu[CPL][4];
+      if (hx) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], nx[j], inv_dx, fxu[j]);
+      }
+      const long r = row_base(x, gy);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        // divergence in the reference order: x, y, z, each (F+ - F-) added in turn
+        T acc[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          bool first = true;
+          if (hx) {
+            acc[a] = fxu[j][a] - fxd[j][a];
+            first = false;
+          }
+          if (hy) {
+            acc[a] = first ? (fyu[j][a] - fyd[j][a]) : ((acc[a] + fyu[j][a]) - fyd[j][a]);
+            first = false;
+          }
+          acc[a] = first ? (fzu[j][a] - fzd[j][a]) : ((acc[a] + fzu[j][a]) - fzd[j][a]);
+        }
+        V4<T> o;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) o.v[a] = acc[a] * inv_dx;
+        if (kon[j]) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
+          st4g<T>(sd.dX + 4 * (r + kz[j]), o);
+        }
+      }
+      if (hx) {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) fxd[j][a] = fxu[j][a];
+          me[j] = nx[j];
+        }
+      }
+    }
+  }
+  if (cur >= 0) flush(cur);
+}
+
 }  // namespace pf5d

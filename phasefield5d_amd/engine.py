@@ -85,9 +85,9 @@ class Engine:
     def variants(self):
         n = C.c_int()
         _lib.check(_lib.lib().pf5d_engine_num_variants(self.h, C.byref(n), None))
-        info = (C.c_int * (3 * n.value))()
+        info = (C.c_int * (5 * n.value))()
         _lib.check(_lib.lib().pf5d_engine_num_variants(self.h, C.byref(n), info))
-        return [tuple(info[3 * i:3 * i + 3]) for i in range(n.value)]
+        return [tuple(info[5 * i:5 * i + 5]) for i in range(n.value)]
 
     def set_tiling(self, variant, cx):
         _lib.check(_lib.lib().pf5d_engine_set_tiling(self.h, int(variant), int(cx)))

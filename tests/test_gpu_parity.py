@@ -129,7 +129,7 @@ def test_elastic_update(dev, dim, shape):
 
 def _split_variant(eng):
     """Index of the two-pass stencil form among the engine's tilings (by == 0)."""
-    return next(i for i, (by, bz, nt) in enumerate(eng.variants()) if by == 0)
+    return next(i for i, v in enumerate(eng.variants()) if v[0] == 0)
 
 
 def _engine_one_step(pb, precision="fp64", form="fused"):
