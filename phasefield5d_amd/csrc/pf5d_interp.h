@@ -232,12 +232,31 @@ PF5D_DEV void interp9_staged(const TableView<T>& tv, const T* box, const NodeBox
   }
 }
 
-// A fixed box of 4^4 table nodes with lower node lo[] (rows of NPAD, node
-// (q0,q1,q2,q3) at row ((q0*4 + q1)*4 + q2)*4 + q3), staged in LDS: points whose
-// lower cube corner lies in its first three nodes on every axis interpolate from
-// it, the others from the table in HBM (same arithmetic either way).  The strides
-// are compile-time constants, so the corner offsets fold into the LDS reads.
+// A fixed box of 4^4 table nodes with lower node lo[], staged in LDS: points
+// whose lower cube corner lies in its first three nodes on every axis
+// interpolate from it, the others from the table in HBM (same arithmetic either
+// way).  Node (q0,q1,q2,q3) starts at 16-byte slot sum_a q_a*S_a.  The strides
+// are compile-time constants, so the corner offsets fold into the LDS reads, and
+// they are chosen so that the 16 nodes of any 2x2x2x2 sub-box start in 16
+// different slots modulo 16 (S3 odd, S2 = 2*S3, S1 = 4*S3, S0 = 8*S3 mod 16):
+// when the lanes of a wavefront interpolate in neighbouring cubes, the reads of
+// one corner hit distinct LDS banks instead of the same few.
 constexpr int BOX4 = 4;
+constexpr int box_stride(int at_least, int residue) {
+  int s = at_least;
+  while ((s & 15) != (residue & 15)) ++s;
+  return s;
+}
+template <typename T>
+struct BoxLayout {
+  static constexpr int NCOL = TablePad<T>::value / Vec16<T>::n;  // 16-byte slots per node (odd)
+  static constexpr int S3 = NCOL;
+  static constexpr int S2 = box_stride(BOX4 * S3, 2 * S3);
+  static constexpr int S1 = box_stride(BOX4 * S2, 4 * S3);
+  static constexpr int S0 = box_stride(BOX4 * S1, 8 * S3);
+  static constexpr int SLOTS = BOX4 * S0;
+  static_assert(NCOL % 2 == 1, "the slot count per node must be odd");
+};
 template <typename T>
 PF5D_DEV void interp9_box4(const TableView<T>& tv, const T* box, const int lo[4], const T X[4], T P[9]) {
   constexpr int NP = TablePad<T>::value;
@@ -256,8 +275,11 @@ PF5D_DEV void interp9_box4(const TableView<T>& tv, const T* box, const int lo[4]
   const int d0 = i0 - lo[0], d1 = i1 - lo[1], d2 = i2 - lo[2], d3 = i3 - lo[3];
   if ((unsigned)d0 <= BOX4 - 2 && (unsigned)d1 <= BOX4 - 2 && (unsigned)d2 <= BOX4 - 2 &&
       (unsigned)d3 <= BOX4 - 2) {
-    const T* base = box + (((d0 * BOX4 + d1) * BOX4 + d2) * BOX4 + d3) * NP;
-    interp_core<T, true>(base, BOX4 * NP, BOX4 * BOX4 * NP, BOX4 * BOX4 * BOX4 * NP, t0, t1, t2, t3, P);
+    using L = BoxLayout<T>;
+    constexpr int VN = Vec16<T>::n;
+    static_assert(L::S3 * VN == NP, "axis-3 stride is one padded row");
+    const T* base = box + (d0 * L::S0 + d1 * L::S1 + d2 * L::S2 + d3 * L::S3) * VN;
+    interp_core<T, true>(base, L::S2 * VN, L::S1 * VN, L::S0 * VN, t0, t1, t2, t3, P);
   } else {
     const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
     interp_core<T, false>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);

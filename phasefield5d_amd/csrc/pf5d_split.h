@@ -48,8 +48,9 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
   constexpr int NP = TablePad<T>::value;
   constexpr int NBOX = BOX4 * BOX4 * BOX4 * BOX4;
   using VT = typename Vec16<T>::type;
-  constexpr int NCOL = NP / Vec16<T>::n;  // 16-byte columns per table row
-  __shared__ __align__(16) T BOX[NBOX * NP];
+  using L = BoxLayout<T>;
+  constexpr int NCOL = L::NCOL;  // 16-byte columns per table row
+  __shared__ __align__(16) T BOX[L::SLOTS * Vec16<T>::n];
   int ib, ie;
   block_range(nitems, ib, ie);
   for (int it = ib; it < ie; ++it) {
@@ -80,7 +81,9 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
       const int node = e / NCOL, col = e - node * NCOL;
       const long row = ((((long)(lo[0] + (node >> 6)) * num + lo[1] + ((node >> 4) & 3)) * num + lo[2] +
                         ((node >> 2) & 3)) * num + lo[3] + (node & 3));
-      reinterpret_cast<VT*>(BOX)[e] = gld<VT>(sd.tab.data + row * NP + col * Vec16<T>::n);
+      const int slot = (node >> 6) * L::S0 + ((node >> 4) & 3) * L::S1 + ((node >> 2) & 3) * L::S2 +
+                       (node & 3) * L::S3 + col;
+      reinterpret_cast<VT*>(BOX)[slot] = gld<VT>(sd.tab.data + row * NP + col * Vec16<T>::n);
     }
     __syncthreads();
     if (valid) {
