@@ -1,8 +1,8 @@
 """Build libpf5d.so in-tree with hipcc for gfx950 (no JIT, no cache outside the tree).
 
-The library is four translation units compiled in parallel -- the host unit
-(C ABI, engine, small kernels), the FFT kernels in FP32 and in FP64, and the
-stencil kernels -- linked into one shared object.
+The library is five translation units compiled in parallel -- the host unit
+(C ABI, engine, small kernels), the FFT line kernels in FP32 and in FP64, the
+FFT plane kernels, and the stencil kernels -- linked into one shared object.
 """
 import os
 import subprocess
@@ -11,7 +11,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-UNITS = ["pf5d_capi.hip", "pf5d_kern_fft32.hip", "pf5d_kern_fft64.hip", "pf5d_kern_stencil.hip"]
+UNITS = ["pf5d_capi.hip", "pf5d_kern_fft32.hip", "pf5d_kern_fft64.hip", "pf5d_kern_fftplane.hip",
+         "pf5d_kern_stencil.hip"]
 OUT = os.path.join(HERE, "libpf5d.so")
 OBJDIR = os.path.join(HERE, "_obj")
 

@@ -256,6 +256,27 @@ static long spectrum_size(int dim, const long* shape) {
   return rows * spec_row<T>(dim, shape);
 }
 
+// plane passes: kernel and LDS bytes for a plane length and a part count
+template <typename T>
+struct PlaneKernels {
+  using Fn = void (*)(const FftPlaneItem*, int, const LinePlanDev*, const cx<T>*, const SimState*);
+  Fn fwd = nullptr, inv = nullptr;
+  size_t fwd_lds = 0, inv_lds = 0;
+};
+template <typename T, int N, int P>
+static PlaneKernels<T> plane_kernels_np() {
+  return {&k_fftp_fwd<T, FFTP_NT, N, P>, &k_fftp_inv<T, FFTP_NT, N, P>, PlaneFwd<N, P>::SLOTS * sizeof(cx<T>),
+          PlaneInv<N, P>::SLOTS * sizeof(cx<T>)};
+}
+// the plane lengths and part counts built; fwd == nullptr when there is none
+template <typename T>
+static PlaneKernels<T> plane_kernels(int n, int parts) {
+  if (n == 128 && parts == 2) return plane_kernels_np<T, 128, 2>();
+  if (n == 128 && parts == 4) return plane_kernels_np<T, 128, 4>();
+  if (n == 128 && parts == 8) return plane_kernels_np<T, 128, 8>();
+  return {};
+}
+
 // the five stages of psi = irfftn(Bhat rfftn(s)) and their work items
 template <typename T>
 struct FftBatch {
@@ -269,6 +290,12 @@ struct FftBatch {
   // tunables: LDS bytes per work item (sets lines per item) and threads per block
   size_t budget = 16 * 1024;
   int nt = 128;
+  // plane passes (3D fields whose y and z lengths are a plane length): parts
+  // per plane, 0 = line passes only
+  int parts = 0;
+  std::vector<FftPlaneItem> pfwd, pinv;
+  FftPlaneItem* d_pitems = nullptr;
+  PlaneKernels<T> pk;
 
   static bool paired(int kind) { return kind == FK_R2C || kind == FK_C2R; }
   // complex lines per item: two line buffers within the LDS budget (real-line
@@ -336,7 +363,9 @@ struct FftBatch {
     b.ls_in = nl;
     b.ps_out = 1;
     b.ls_out = nhp;
+    size_t n0 = stage[0].size();
     add_lines(0, b, nl, rows, sizeof(T) * nl, sizeof(cx<T>) * nhp, 0);
+    const size_t nrows_r2c = stage[0].size() - n0;
     // rows: C2R
     b.kind = FK_C2R;
     b.in = C;
@@ -345,7 +374,9 @@ struct FftBatch {
     b.ls_in = nhp;
     b.ps_out = 1;
     b.ls_out = nl;
+    n0 = stage[4].size();
     add_lines(4, b, nl, rows, sizeof(cx<T>) * nhp, sizeof(T) * nl, 0);
+    const size_t nrows_c2r = stage[4].size() - n0;
     if (dim == 2) {
       const int nx = (int)shape[0];
       b.kind = FK_CONV;
@@ -359,6 +390,19 @@ struct FftBatch {
       return;
     }
     const int nx = (int)shape[0], ny = (int)shape[1];
+    const PlaneKernels<T> k = (ny == nl) ? plane_kernels<T>(nl, parts) : PlaneKernels<T>{};
+    if (k.fwd && k.fwd_lds <= (size_t)devinfo().lds && k.inv_lds <= (size_t)devinfo().lds) {
+      // rows and y lines of each plane in the plane passes; the row stages above are dropped
+      stage[0].resize(stage[0].size() - nrows_r2c);
+      stage[4].resize(stage[4].size() - nrows_c2r);
+      pk = k;
+      const int plan = plans.get(nl);
+      for (int x = 0; x < nx; ++x)
+        for (int p = 0; p < parts; ++p) {
+          pfwd.push_back({sim, plan, x, p, s, C, (int)nhp});
+          pinv.push_back({sim, plan, x, p, C, psi, (int)nhp});
+        }
+    } else {
     // y passes: for each x plane, lines kz in [0, nh) with point stride nhp
     for (int x = 0; x < nx; ++x) {
       FftItem y{};
@@ -372,6 +416,7 @@ struct FftBatch {
       add_lines(1, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
       y.kind = FK_INV;
       add_lines(3, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
+    }
     }
     // x pass with Bhat: for each y, lines kz in [0, nh) with point stride ny*nhp
     for (int y = 0; y < ny; ++y) {
@@ -389,10 +434,21 @@ struct FftBatch {
   void clear() {
     for (auto& s : stage) s.clear();
     for (auto& l : lds) l = 0;
+    pfwd.clear();
+    pinv.clear();
+    pk = {};
   }
   void upload() {
     dfree(d_items);
     d_items = nullptr;
+    dfree(d_pitems);
+    d_pitems = nullptr;
+    if (!pfwd.empty()) {
+      std::vector<FftPlaneItem> pall(pfwd);
+      pall.insert(pall.end(), pinv.begin(), pinv.end());
+      d_pitems = (FftPlaneItem*)dmalloc(pall.size() * sizeof(FftPlaneItem));
+      HIPCHK(hipMemcpy(d_pitems, pall.data(), pall.size() * sizeof(FftPlaneItem), hipMemcpyHostToDevice));
+    }
     std::vector<FftItem> all;
     for (int i = 0; i < 5; ++i) {
       offs[i] = (int)all.size();
@@ -403,8 +459,18 @@ struct FftBatch {
     d_items = (FftItem*)dmalloc(all.size() * sizeof(FftItem));
     HIPCHK(hipMemcpy(d_items, all.data(), all.size() * sizeof(FftItem), hipMemcpyHostToDevice));
   }
+  void run_planes(hipStream_t stream, const SimState* st, bool inverse) {
+    const int n = (int)pfwd.size();
+    if (n == 0) return;
+    const DevInfo& di = devinfo();
+    const unsigned grid = xcd_grid(std::min<long>(n, (long)di.cus * 8));
+    hipLaunchKernelGGL(inverse ? pk.inv : pk.fwd, dim3(grid), dim3(FFTP_NT), inverse ? pk.inv_lds : pk.fwd_lds, stream,
+                       d_pitems + (inverse ? n : 0), n, plans.d_plans, plans.d_tw, st);
+    HIPCHK(hipGetLastError());
+  }
   void run(hipStream_t stream, const SimState* st) {
     const DevInfo& di = devinfo();
+    run_planes(stream, st, false);
     for (int i = 0; i < 5; ++i) {
       const int n = offs[i + 1] - offs[i];
       if (n == 0) continue;
@@ -424,8 +490,12 @@ struct FftBatch {
                            plans.d_plans, plans.d_tw, plans.d_roots, st);
       HIPCHK(hipGetLastError());
     }
+    run_planes(stream, st, true);
   }
-  ~FftBatch() { dfree(d_items); }
+  ~FftBatch() {
+    dfree(d_items);
+    dfree(d_pitems);
+  }
 };
 
 // Bhat (k_shape, double) -> device T in the spectrum layout, with 1/N folded in
@@ -514,6 +584,7 @@ struct EngineBase {
   virtual void stats(long* active, long* bytes) = 0;
   virtual void set_tiling(int variant, int cx) = 0;
   virtual void set_fft(int budget_kb, int nt) = 0;
+  virtual void set_fft_planes(int parts) = 0;
   virtual int num_variants(int* info) = 0;
   virtual void launch_info(long* out) = 0;
   virtual double time_kernel(int which, int reps) = 0;
@@ -634,6 +705,11 @@ struct Engine : EngineBase {
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+    for (int parts : {2, 4, 8}) {
+      const PlaneKernels<T> k = plane_kernels<T>(128, parts);
+      (void)hipFuncSetAttribute((const void*)k.fwd, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+      (void)hipFuncSetAttribute((const void*)k.inv, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+    }
   }
   ~Engine() override {
     drop_graphs();
@@ -1081,6 +1157,11 @@ struct Engine : EngineBase {
     xchunk = c;
     dirty = true;
   }
+  void set_fft_planes(int parts) override {
+    require(parts == 0 || parts == 2 || parts == 4 || parts == 8, "FFT plane parts must be 0, 2, 4 or 8");
+    fft.parts = parts;
+    dirty = true;
+  }
   void set_fft(int budget_kb, int nt) override {
     require(budget_kb >= 4 && budget_kb <= 160, "FFT LDS budget must be 4..160 KiB");
     require(nt == 64 || nt == 128 || nt == 256 || nt == 512, "FFT block size must be 64, 128, 256 or 512");
@@ -1143,6 +1224,7 @@ struct Engine : EngineBase {
     out[7] = (long)fft.budget;
     out[8] = fft.nt;
     out[9] = kf_grid;
+    out[10] = fft.pfwd.empty() ? 0 : fft.parts;
   }
 };
 
@@ -1526,6 +1608,9 @@ int pf5d_engine_remove(pf5d_engine* e, int slot) {
 }
 int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* info) {
   return guarded([&] { *n = e->impl->num_variants(info); });
+}
+int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts) {
+  return guarded([&] { e->impl->set_fft_planes(parts); });
 }
 int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt) {
   return guarded([&] { e->impl->set_fft(budget_kb, nt); });

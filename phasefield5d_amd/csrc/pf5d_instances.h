@@ -6,6 +6,7 @@
 // kernel units define them (PF5D_INST = template).
 #pragma once
 #include "pf5d_fft.h"
+#include "pf5d_fftplane.h"
 #include "pf5d_fused.h"
 #include "pf5d_split.h"
 
@@ -19,6 +20,13 @@ namespace pf5d {
   PF5D_INST __global__ void k_fft<T, NT>(const FftItem* __restrict__, int, const LinePlanDev* __restrict__, \
                                          const cx<T>* __restrict__, const cx<T>* __restrict__,                 \
                                          const SimState* __restrict__);
+#define PF5D_FFTP_INSTANCE(T, N, P)                                                                             \
+  PF5D_INST __global__ void k_fftp_fwd<T, FFTP_NT, N, P>(const FftPlaneItem* __restrict__, int,                 \
+                                                         const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
+                                                         const SimState* __restrict__);                        \
+  PF5D_INST __global__ void k_fftp_inv<T, FFTP_NT, N, P>(const FftPlaneItem* __restrict__, int,                 \
+                                                         const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
+                                                         const SimState* __restrict__);
 #define PF5D_STENCIL_INSTANCE(T, BY, BZ, NT, MINW)                                                 \
   PF5D_INST __global__ void k_stencil<T, BY, BZ, NT, MINW>(const SimDev<T>* __restrict__,          \
                                                            SimState* __restrict__, T* __restrict__, \
@@ -43,6 +51,15 @@ namespace pf5d {
   PF5D_FFT_INSTANCE(double, 128)  \
   PF5D_FFT_INSTANCE(double, 256)  \
   PF5D_FFT_INSTANCE(double, 512)
+
+// plane passes: lengths and part counts (the FP64 inverse with 2 parts needs more LDS than a block has)
+#define PF5D_FFTP_INSTANCES       \
+  PF5D_FFTP_INSTANCE(float, 128, 2)  \
+  PF5D_FFTP_INSTANCE(float, 128, 4)  \
+  PF5D_FFTP_INSTANCE(float, 128, 8)  \
+  PF5D_FFTP_INSTANCE(double, 128, 2) \
+  PF5D_FFTP_INSTANCE(double, 128, 4) \
+  PF5D_FFTP_INSTANCE(double, 128, 8)
 
 // the stencil tilings and two-pass forms the engine offers the autotuner
 #define PF5D_STENCIL_INSTANCES                    \

@@ -92,8 +92,12 @@ class Engine:
     def set_tiling(self, variant, cx):
         _lib.check(_lib.lib().pf5d_engine_set_tiling(self.h, int(variant), int(cx)))
 
-    def set_fft(self, budget_kb, nt):
+    def set_fft(self, budget_kb, nt, parts=None):
+        """Line-pass batching (LDS KiB per item, threads per block) and, when given,
+        the parts per plane of the 3D plane passes (0 = line passes only)."""
         _lib.check(_lib.lib().pf5d_engine_set_fft(self.h, int(budget_kb), int(nt)))
+        if parts is not None:
+            _lib.check(_lib.lib().pf5d_engine_set_fft_planes(self.h, int(parts)))
 
     # -- readback ----------------------------------------------------------------
     def state(self, slot):
@@ -156,7 +160,7 @@ class Engine:
         return ms.value
 
     def launch_info(self):
-        out = (C.c_long * 10)()
+        out = (C.c_long * 11)()
         _lib.check(_lib.lib().pf5d_engine_launch_info(self.h, out))
         return dict(zip(("k1_items", "k2_items", "fft_items", "k1_grid", "k1_lds", "variant", "cx", "fft_budget",
-                         "fft_nt", "kf_grid"), list(out)))
+                         "fft_nt", "kf_grid", "fft_parts"), list(out)))

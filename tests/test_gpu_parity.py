@@ -311,6 +311,30 @@ def test_stencil_forms_bitwise(dev, precision):
             assert np.array_equal(Xa, Xb) and np.array_equal(da, db) and ta == tb, f"variant {v}"
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fft_plane_passes_bitwise(dev, precision):
+    """The plane passes (z and y transforms of one x-plane in one work item, the
+    plane split into 2, 4 or 8 parts) give the same bits as the five line passes;
+    where a part count needs more LDS than a block has, the line passes run."""
+    from phasefield5d_amd.engine import Engine
+    pb = problem(3, 128, seed=9)
+    pb.X_init = developed_state(pb, amp=0.03)
+    out, used = {}, {}
+    for parts in (0, 2, 4, 8):
+        e = Engine(precision)
+        s = e.add_problem(pb)
+        e.set_fft(16, 128, parts)
+        e.step(2)
+        out[parts] = (e.field(s, "X"), e.field(s, "psi"), e.state(s)["dt"])
+        used[parts] = e.launch_info()["fft_parts"]
+        e.close()
+    assert used[0] == 0 and used[4] == 4 and used[8] == 8
+    assert used[2] == (2 if precision == "fp32" else 0)
+    for parts in (2, 4, 8):
+        for a, b in zip(out[0], out[parts]):
+            assert np.array_equal(a, b), f"parts {parts}"
+
+
 def test_batched_equals_alone(dev):
     """Each simulation's result is bit-identical alone and inside a mixed batch."""
     from phasefield5d_amd.engine import Engine
