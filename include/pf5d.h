@@ -146,9 +146,10 @@ int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx);
 int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt);
 /* FFT plane passes for 3D fields whose y and z lengths are a plane length (128):
  * the z and y transforms of an x-plane in one work item, the plane split into
- * parts (2, 4 or 8) work items; 0 = line passes only (the default).  Where the
- * plane passes need more LDS than a block has, the line passes are used. */
-int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts);
+ * parts (2, 4 or 8) work items of nt threads (256, 512 or 1024); parts = 0: line
+ * passes only (the default).  Where the plane passes need more LDS than a block
+ * has, the line passes are used. */
+int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts, int nt);
 /* enqueue nsteps (graph replay); does not block */
 int pf5d_engine_step(pf5d_engine* e, int nsteps);
 int pf5d_engine_sync(pf5d_engine* e);
@@ -168,8 +169,8 @@ void* pf5d_engine_stream(pf5d_engine* e);
 int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes);
 /* launch configuration of the current work lists, for profiling reports: [K1 items, K2 items,
  * FFT items, K1 grid, K1 LDS bytes, stencil variant, x chunk, FFT LDS budget bytes, FFT block
- * size, flux-pass grid, FFT plane parts in use (0: line passes only)] */
-int pf5d_engine_launch_info(pf5d_engine* e, long out[11]);
+ * size, flux-pass grid, FFT plane parts in use (0: line passes only), FFT plane block size] */
+int pf5d_engine_launch_info(pf5d_engine* e, long out[12]);
 
 /* average device time of one launch of a stage (HIP events on the engine stream):
  * which = 0 stencil stage K1 (either form), 1 the FFT passes, 2 dt controller, 3 update

@@ -262,18 +262,26 @@ struct PlaneKernels {
   using Fn = void (*)(const FftPlaneItem*, int, const LinePlanDev*, const cx<T>*, const SimState*);
   Fn fwd = nullptr, inv = nullptr;
   size_t fwd_lds = 0, inv_lds = 0;
+  int nt = 0;
 };
-template <typename T, int N, int P>
+template <typename T, int NT, int N, int P>
 static PlaneKernels<T> plane_kernels_np() {
-  return {&k_fftp_fwd<T, FFTP_NT, N, P>, &k_fftp_inv<T, FFTP_NT, N, P>, PlaneFwd<N, P>::SLOTS * sizeof(cx<T>),
-          PlaneInv<N, P>::SLOTS * sizeof(cx<T>)};
+  return {&k_fftp_fwd<T, NT, N, P>, &k_fftp_inv<T, NT, N, P>, PlaneFwd<T, N, P>::SLOTS * sizeof(cx<T>),
+          PlaneInv<T, N, P>::SLOTS * sizeof(cx<T>)};
 }
-// the plane lengths and part counts built; fwd == nullptr when there is none
+template <typename T, int NT>
+static PlaneKernels<T> plane_kernels_nt(int n, int parts) {
+  if (n == 128 && parts == 2) return plane_kernels_np<T, NT, 128, 2>();
+  if (n == 128 && parts == 4) return plane_kernels_np<T, NT, 128, 4>();
+  if (n == 128 && parts == 8) return plane_kernels_np<T, NT, 128, 8>();
+  return {};
+}
+// the block sizes, plane lengths and part counts built; fwd == nullptr when there is none
 template <typename T>
-static PlaneKernels<T> plane_kernels(int n, int parts) {
-  if (n == 128 && parts == 2) return plane_kernels_np<T, 128, 2>();
-  if (n == 128 && parts == 4) return plane_kernels_np<T, 128, 4>();
-  if (n == 128 && parts == 8) return plane_kernels_np<T, 128, 8>();
+static PlaneKernels<T> plane_kernels(int n, int parts, int nt) {
+  if (nt == 256) return plane_kernels_nt<T, 256>(n, parts);
+  if (nt == 512) return plane_kernels_nt<T, 512>(n, parts);
+  if (nt == 1024) return plane_kernels_nt<T, 1024>(n, parts);
   return {};
 }
 
@@ -292,7 +300,7 @@ struct FftBatch {
   int nt = 128;
   // plane passes (3D fields whose y and z lengths are a plane length): parts
   // per plane, 0 = line passes only
-  int parts = 0;
+  int parts = 0, pnt = 256;  // parts per plane (0 = line passes only), threads per block
   std::vector<FftPlaneItem> pfwd, pinv;
   FftPlaneItem* d_pitems = nullptr;
   PlaneKernels<T> pk;
@@ -390,7 +398,8 @@ struct FftBatch {
       return;
     }
     const int nx = (int)shape[0], ny = (int)shape[1];
-    const PlaneKernels<T> k = (ny == nl) ? plane_kernels<T>(nl, parts) : PlaneKernels<T>{};
+    PlaneKernels<T> k = (ny == nl) ? plane_kernels<T>(nl, parts, pnt) : PlaneKernels<T>{};
+    k.nt = pnt;
     if (k.fwd && k.fwd_lds <= (size_t)devinfo().lds && k.inv_lds <= (size_t)devinfo().lds) {
       // rows and y lines of each plane in the plane passes; the row stages above are dropped
       stage[0].resize(stage[0].size() - nrows_r2c);
@@ -464,7 +473,7 @@ struct FftBatch {
     if (n == 0) return;
     const DevInfo& di = devinfo();
     const unsigned grid = xcd_grid(std::min<long>(n, (long)di.cus * 8));
-    hipLaunchKernelGGL(inverse ? pk.inv : pk.fwd, dim3(grid), dim3(FFTP_NT), inverse ? pk.inv_lds : pk.fwd_lds, stream,
+    hipLaunchKernelGGL(inverse ? pk.inv : pk.fwd, dim3(grid), dim3(pk.nt), inverse ? pk.inv_lds : pk.fwd_lds, stream,
                        d_pitems + (inverse ? n : 0), n, plans.d_plans, plans.d_tw, st);
     HIPCHK(hipGetLastError());
   }
@@ -584,7 +593,7 @@ struct EngineBase {
   virtual void stats(long* active, long* bytes) = 0;
   virtual void set_tiling(int variant, int cx) = 0;
   virtual void set_fft(int budget_kb, int nt) = 0;
-  virtual void set_fft_planes(int parts) = 0;
+  virtual void set_fft_planes(int parts, int nt) = 0;
   virtual int num_variants(int* info) = 0;
   virtual void launch_info(long* out) = 0;
   virtual double time_kernel(int which, int reps) = 0;
@@ -705,11 +714,12 @@ struct Engine : EngineBase {
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
-    for (int parts : {2, 4, 8}) {
-      const PlaneKernels<T> k = plane_kernels<T>(128, parts);
-      (void)hipFuncSetAttribute((const void*)k.fwd, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
-      (void)hipFuncSetAttribute((const void*)k.inv, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
-    }
+    for (int pnt : {256, 512, 1024})
+      for (int parts : {2, 4, 8}) {
+        const PlaneKernels<T> k = plane_kernels<T>(128, parts, pnt);
+        (void)hipFuncSetAttribute((const void*)k.fwd, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+        (void)hipFuncSetAttribute((const void*)k.inv, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+      }
   }
   ~Engine() override {
     drop_graphs();
@@ -1157,9 +1167,11 @@ struct Engine : EngineBase {
     xchunk = c;
     dirty = true;
   }
-  void set_fft_planes(int parts) override {
+  void set_fft_planes(int parts, int nt) override {
     require(parts == 0 || parts == 2 || parts == 4 || parts == 8, "FFT plane parts must be 0, 2, 4 or 8");
+    require(nt == 256 || nt == 512 || nt == 1024, "FFT plane block size must be 256, 512 or 1024");
     fft.parts = parts;
+    fft.pnt = nt;
     dirty = true;
   }
   void set_fft(int budget_kb, int nt) override {
@@ -1225,6 +1237,7 @@ struct Engine : EngineBase {
     out[8] = fft.nt;
     out[9] = kf_grid;
     out[10] = fft.pfwd.empty() ? 0 : fft.parts;
+    out[11] = fft.pnt;
   }
 };
 
@@ -1609,8 +1622,8 @@ int pf5d_engine_remove(pf5d_engine* e, int slot) {
 int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* info) {
   return guarded([&] { *n = e->impl->num_variants(info); });
 }
-int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts) {
-  return guarded([&] { e->impl->set_fft_planes(parts); });
+int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts, int nt) {
+  return guarded([&] { e->impl->set_fft_planes(parts, nt); });
 }
 int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt) {
   return guarded([&] { e->impl->set_fft(budget_kb, nt); });
@@ -1654,7 +1667,7 @@ int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes) {
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms) {
   return guarded([&] { *avg_ms = e->impl->time_kernel(which, reps); });
 }
-int pf5d_engine_launch_info(pf5d_engine* e, long out[10]) {
+int pf5d_engine_launch_info(pf5d_engine* e, long out[12]) {
   return guarded([&] { e->impl->launch_info(out); });
 }
 

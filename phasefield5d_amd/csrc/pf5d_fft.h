@@ -39,7 +39,17 @@ PF5D_DEV cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
 PF5D_DEV cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
 template <typename T>
-PF5D_DEV cx<T> cmul(cx<T> a, cx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+PF5D_DEV T fused_mad(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_fmaf(a, b, c);
+  else
+    return __builtin_fma(a, b, c);
+}
+// two multiplies and two fused multiply-adds
+template <typename T>
+PF5D_DEV cx<T> cmul(cx<T> a, cx<T> b) {
+  return {fused_mad<T>(a.x, b.x, -(a.y * b.y)), fused_mad<T>(a.x, b.y, a.y * b.x)};
+}
 template <typename T>
 PF5D_DEV cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
 template <typename T>

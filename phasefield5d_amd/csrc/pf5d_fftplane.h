@@ -23,8 +23,6 @@
 
 namespace pf5d {
 
-constexpr int FFTP_NT = 256;  // threads per block of the plane passes
-
 struct FftPlaneItem {
   int sim, plan, x, part;
   const void* in;  // forward: s (real, n0 x N x N); inverse: spectrum (rows of nhp)
@@ -32,33 +30,51 @@ struct FftPlaneItem {
   int nhp;         // spectrum row length
 };
 
-// largest power of two that divides m and is at most cap (>= 1)
-PF5D_HD constexpr int pow2_div_upto(int m, int cap) {
-  int g = 1;
-  while (m % (2 * g) == 0 && 2 * g <= cap) g *= 2;
-  return g;
-}
+// LDS of one block on gfx950; the batch sizes below are the largest that fit
+constexpr long FFTP_LDS_CAP = 160 * 1024;
 
-template <int N, int P>
+template <typename T, int N, int P>
 struct PlaneFwd {
   static constexpr int NH = N / 2 + 1, LS = line_stride(N);
   static constexpr int KC = (NH + P - 1) / P;  // spectrum columns of the largest part
-  // row pairs per batch of the rows stage: two batch buffers fit the pass buffer
-  static constexpr int GB = pow2_div_upto(N / 2, KC / 2 > 0 ? KC / 2 : 1);
+  // LDS slots with batches of g row pairs: twiddles, the kept columns, the pass
+  // buffer (two batch buffers, or the kept columns' second buffer)
+  static constexpr int slots(int g) { return N + KC * LS + ((KC > 2 * g) ? KC : 2 * g) * LS; }
+  // row pairs per batch: the largest power of two dividing N/2 that fits
+  static constexpr int pick() {
+    int g = 1;
+    while ((N / 2) % (2 * g) == 0 && (long)slots(2 * g) * (long)sizeof(cx<T>) <= FFTP_LDS_CAP) g *= 2;
+    return g;
+  }
+  static constexpr int GB = pick();
   static constexpr int BL = (KC > 2 * GB) ? KC : 2 * GB;  // lines of the pass buffer
-  // LDS slots: twiddles, the kept columns, the pass buffer
-  static constexpr int SLOTS = N + KC * LS + BL * LS;
+  static constexpr int SLOTS = slots(GB);
 };
 
-template <int N, int P>
+template <typename T, int N, int P>
 struct PlaneInv {
   static constexpr int NH = N / 2 + 1, LS = line_stride(N), LSR = line_stride(NH);
   static constexpr int RP = (N / 2 + P - 1) / P;  // row pairs of the largest part
-  static constexpr int NB = (NH + RP - 1) / RP;   // column batches of the y stage
-  static constexpr int GC = (NH + NB - 1) / NB;   // columns per batch
-  static constexpr int BL = (RP > GC) ? RP : GC;  // lines of each pass buffer
-  // LDS slots: twiddles, the kept rows (2 RP rows of NH), two pass buffers
-  static constexpr int SLOTS = N + 2 * RP * LSR + 2 * BL * LS;
+  static constexpr int NPASS = fft_npass(N);
+  // LDS slots with column batches of c lines: twiddles, the kept rows (2 RP rows
+  // of NH), the pass buffers (the y stage's one or two of c lines, the rows
+  // stage's two of RP lines)
+  static constexpr int blines(int c) {
+    const int y = (NPASS > 2) ? 2 * c : c;
+    return (y > 2 * RP) ? y : 2 * RP;
+  }
+  static constexpr int slots(int c) { return N + 2 * RP * LSR + blines(c) * LS; }
+  // columns per batch: as few batches as fit, evenly sized
+  static constexpr int pick() {
+    for (int nb = 1; nb < NH; ++nb) {
+      const int c = (NH + nb - 1) / nb;
+      if ((long)slots(c) * (long)sizeof(cx<T>) <= FFTP_LDS_CAP) return c;
+    }
+    return 1;
+  }
+  static constexpr int GC = pick();
+  static constexpr int BL = blines(GC);  // lines of the pass buffer region
+  static constexpr int SLOTS = slots(GC);
 };
 
 template <typename T, int NT>
@@ -70,7 +86,7 @@ template <typename T, int NT, int N, int P>
 __global__ void __launch_bounds__(NT) k_fftp_fwd(const FftPlaneItem* __restrict__ items, int nitems,
                                                  const LinePlanDev* __restrict__ plans,
                                                  const cx<T>* __restrict__ tw, const SimState* __restrict__ st) {
-  using S = PlaneFwd<N, P>;
+  using S = PlaneFwd<T, N, P>;
   constexpr int LS = S::LS, NH = S::NH, GB = S::GB;
   constexpr int PF = (2 * GB * N + NT - 1) / NT;  // row points per thread per batch
   extern __shared__ __align__(16) unsigned char smem[];
@@ -151,13 +167,15 @@ template <typename T, int NT, int N, int P>
 __global__ void __launch_bounds__(NT) k_fftp_inv(const FftPlaneItem* __restrict__ items, int nitems,
                                                  const LinePlanDev* __restrict__ plans,
                                                  const cx<T>* __restrict__ tw, const SimState* __restrict__ st) {
-  using S = PlaneInv<N, P>;
+  using S = PlaneInv<T, N, P>;
   constexpr int LS = S::LS, LSR = S::LSR, NH = S::NH, GC = S::GC;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* twl = reinterpret_cast<cx<T>*>(smem);
   cx<T>* PL = twl + N;               // kept rows: row i = y0 + i, point kz
   cx<T>* B0 = PL + 2 * S::RP * LSR;  // pass buffers
-  cx<T>* B1 = B0 + S::BL * LS;
+  // the y stage's second buffer (plans of three or more passes), the rows stage's second buffer
+  cx<T>* B1y = B0 + GC * LS;
+  cx<T>* B1r = B0 + S::RP * LS;
   int staged = -1;
   int ib, ie;
   block_range(nitems, ib, ie);
@@ -184,7 +202,7 @@ __global__ void __launch_bounds__(NT) k_fftp_inv(const FftPlaneItem* __restrict_
       auto sink_pl = [&](int g, int m, const cx<T>& v, cx<T>*) {
         if (m >= y0 && m < y1) PL[(m - y0) * LSR + lpad(c0 + g)] = conjg(v);
       };
-      CtRun<T, NT, N, 0, 1>::run(gc, 1.0f / (float)gc, tid, twl, B0, B1, true, fetch_c, false, sink_pl);
+      CtRun<T, NT, N, 0, 1>::run(gc, 1.0f / (float)gc, tid, twl, B0, B1y, true, fetch_c, false, sink_pl);
       __syncthreads();
     }
     // rows stage (C2R) on the kept row pairs: complex line g carries
@@ -198,9 +216,9 @@ __global__ void __launch_bounds__(NT) k_fftp_inv(const FftPlaneItem* __restrict_
       return {ca.x + cb.y, ca.y - cb.x};
     };
     auto natural = [&](int g, int m, const cx<T>& v, cx<T>* other) { other[g * LS + lpad(m)] = v; };
-    CtRun<T, NT, N, 0, 1>::run(np, 1.0f / (float)np, tid, twl, B0, B1, false, fetch_pl, false, natural);
+    CtRun<T, NT, N, 0, 1>::run(np, 1.0f / (float)np, tid, twl, B0, B1r, false, fetch_pl, false, natural);
     __syncthreads();
-    const cx<T>* res = (fft_npass(N) % 2 == 1) ? B0 : B1;
+    const cx<T>* res = (fft_npass(N) % 2 == 1) ? B0 : B1r;
     // C2R: Re to row y0 + 2g, -Im to row y0 + 2g + 1, point-contiguous
     T* psi = reinterpret_cast<T*>(I.out) + ((long)I.x * N + y0) * N;
 #pragma unroll 1

@@ -20,13 +20,13 @@ namespace pf5d {
   PF5D_INST __global__ void k_fft<T, NT>(const FftItem* __restrict__, int, const LinePlanDev* __restrict__, \
                                          const cx<T>* __restrict__, const cx<T>* __restrict__,                 \
                                          const SimState* __restrict__);
-#define PF5D_FFTP_INSTANCE(T, N, P)                                                                             \
-  PF5D_INST __global__ void k_fftp_fwd<T, FFTP_NT, N, P>(const FftPlaneItem* __restrict__, int,                 \
-                                                         const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
-                                                         const SimState* __restrict__);                        \
-  PF5D_INST __global__ void k_fftp_inv<T, FFTP_NT, N, P>(const FftPlaneItem* __restrict__, int,                 \
-                                                         const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
-                                                         const SimState* __restrict__);
+#define PF5D_FFTP_INSTANCE(T, NT, N, P)                                                                    \
+  PF5D_INST __global__ void k_fftp_fwd<T, NT, N, P>(const FftPlaneItem* __restrict__, int,                 \
+                                                    const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
+                                                    const SimState* __restrict__);                        \
+  PF5D_INST __global__ void k_fftp_inv<T, NT, N, P>(const FftPlaneItem* __restrict__, int,                 \
+                                                    const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
+                                                    const SimState* __restrict__);
 #define PF5D_STENCIL_INSTANCE(T, BY, BZ, NT, MINW)                                                 \
   PF5D_INST __global__ void k_stencil<T, BY, BZ, NT, MINW>(const SimDev<T>* __restrict__,          \
                                                            SimState* __restrict__, T* __restrict__, \
@@ -52,14 +52,21 @@ namespace pf5d {
   PF5D_FFT_INSTANCE(double, 256)  \
   PF5D_FFT_INSTANCE(double, 512)
 
-// plane passes: lengths and part counts (the FP64 inverse with 2 parts needs more LDS than a block has)
-#define PF5D_FFTP_INSTANCES       \
-  PF5D_FFTP_INSTANCE(float, 128, 2)  \
-  PF5D_FFTP_INSTANCE(float, 128, 4)  \
-  PF5D_FFTP_INSTANCE(float, 128, 8)  \
-  PF5D_FFTP_INSTANCE(double, 128, 2) \
-  PF5D_FFTP_INSTANCE(double, 128, 4) \
-  PF5D_FFTP_INSTANCE(double, 128, 8)
+// plane passes: block sizes, lengths and part counts (the FP64 inverse with 2
+// parts needs more LDS than a block has; the host then runs the line passes)
+#define PF5D_FFTP_INSTANCES_T(T)     \
+  PF5D_FFTP_INSTANCE(T, 256, 128, 2)  \
+  PF5D_FFTP_INSTANCE(T, 256, 128, 4)  \
+  PF5D_FFTP_INSTANCE(T, 256, 128, 8)  \
+  PF5D_FFTP_INSTANCE(T, 512, 128, 2)  \
+  PF5D_FFTP_INSTANCE(T, 512, 128, 4)  \
+  PF5D_FFTP_INSTANCE(T, 512, 128, 8)  \
+  PF5D_FFTP_INSTANCE(T, 1024, 128, 2) \
+  PF5D_FFTP_INSTANCE(T, 1024, 128, 4) \
+  PF5D_FFTP_INSTANCE(T, 1024, 128, 8)
+#define PF5D_FFTP_INSTANCES  \
+  PF5D_FFTP_INSTANCES_T(float) \
+  PF5D_FFTP_INSTANCES_T(double)
 
 // the stencil tilings and two-pass forms the engine offers the autotuner
 #define PF5D_STENCIL_INSTANCES                    \

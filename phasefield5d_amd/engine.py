@@ -92,12 +92,13 @@ class Engine:
     def set_tiling(self, variant, cx):
         _lib.check(_lib.lib().pf5d_engine_set_tiling(self.h, int(variant), int(cx)))
 
-    def set_fft(self, budget_kb, nt, parts=None):
+    def set_fft(self, budget_kb, nt, parts=None, plane_nt=256):
         """Line-pass batching (LDS KiB per item, threads per block) and, when given,
-        the parts per plane of the 3D plane passes (0 = line passes only)."""
+        the parts per plane of the 3D plane passes (0 = line passes only) and
+        their block size."""
         _lib.check(_lib.lib().pf5d_engine_set_fft(self.h, int(budget_kb), int(nt)))
         if parts is not None:
-            _lib.check(_lib.lib().pf5d_engine_set_fft_planes(self.h, int(parts)))
+            _lib.check(_lib.lib().pf5d_engine_set_fft_planes(self.h, int(parts), int(plane_nt)))
 
     # -- readback ----------------------------------------------------------------
     def state(self, slot):
@@ -160,7 +161,7 @@ class Engine:
         return ms.value
 
     def launch_info(self):
-        out = (C.c_long * 11)()
+        out = (C.c_long * 12)()
         _lib.check(_lib.lib().pf5d_engine_launch_info(self.h, out))
         return dict(zip(("k1_items", "k2_items", "fft_items", "k1_grid", "k1_lds", "variant", "cx", "fft_budget",
-                         "fft_nt", "kf_grid", "fft_parts"), list(out)))
+                         "fft_nt", "kf_grid", "fft_parts", "fft_plane_nt"), list(out)))
