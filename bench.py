@@ -72,7 +72,7 @@ def batch_leg(steps, warmup, log_fn):
     eng = Engine("fp32")
     for pb in pbs:
         eng.add_problem(pb)
-    tuned = autotune.tune(eng, chunks=(16,), reps=3)
+    tuned = autotune.tune(eng, chunks=(8, 16), reps=3)
     eng.step(warmup)
     eng.sync()
     torch.cuda.synchronize()
@@ -149,7 +149,7 @@ def main():
             eng.set_tiling(args.variant or 0, args.cx or 16)
         else:
             from phasefield5d_amd import autotune
-            tuned = autotune.tune(eng, chunks=(16, 32), reps=5)
+            tuned = autotune.tune(eng, chunks=(4, 8, 16, 32), reps=5)
             log(f"{precision} autotuned: {tuned}")
         eng.step(warmup)
         eng.sync()

@@ -1,12 +1,16 @@
-"""Autotuner for the stencil stage (and, through it, the FFT line batching).
+"""Autotuner for the stencil stage and the FFT passes.
 
 For an engine holding its simulations, every compiled K1 variant -- the
 single-pass fused tilings (BY x BZ tile, threads, register budget) combined
-with x-chunk lengths for 3D marching, and the two-pass form (reported as
-tile 0 x 64) -- is timed with HIP events on the engine stream
-(``Engine.time_kernel``); the fastest is kept.  Results are cached per
-(precision, multiset of simulation shapes) in a JSON file next to the library
-so later engines with the same mix start tuned.
+with x-chunk lengths for 3D marching, and the two-pass forms (reported as
+tile 0 x 64, with the flux pass's cells per lane and block size; the marching
+flux pass takes x-chunk lengths too) -- is timed with HIP events on the engine
+stream (``Engine.time_kernel``); the fastest is kept.  Then the FFT: the line
+passes' batching (LDS per item, block size), the plane passes of 3D fields
+with 128 x 128 planes (parts per plane, block size), and the line batching
+again for the x pass the planes leave.  Results are cached per (precision,
+multiset of simulation shapes) in a JSON file next to the library so later
+engines with the same mix start tuned.
 """
 import json
 import os
@@ -20,7 +24,7 @@ _lock = threading.Lock()
 
 def signature(engine):
     shapes = sorted(tuple(pb.shape) for pb, _ in engine.slots.values())
-    return f"{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
+    return f"v3:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
 
 
 def _load():
@@ -41,8 +45,12 @@ def _store(cache):
         pass
 
 
-def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft_budgets=(8, 16, 32, 64, 160)):
-    """Pick the fastest (variant, x-chunk) and FFT batching for the engine's current simulations."""
+def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft_budgets=(8, 16, 32, 64, 160),
+         quick=False):
+    """Pick the fastest (variant, x-chunk) and FFT batching for the engine's current simulations.
+
+    ``quick`` narrows the FFT search to a few line and plane settings (for callers
+    that tune on every change of their mix, such as the server)."""
     sig = signature(engine)
     with _lock:
         cache = _load() if use_cache else {}
@@ -50,7 +58,7 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
         ent = cache[sig]
         engine.set_tiling(ent["variant"], ent["cx"])
         if "fft_kb" in ent:
-            engine.set_fft(ent["fft_kb"], ent["fft_nt"])
+            engine.set_fft(ent["fft_kb"], ent["fft_nt"], ent.get("fft_parts", 0), ent.get("fft_plane_nt", 256))
         return ent
     has3d = any(len(pb.shape) == 3 for pb, _ in engine.slots.values())
     results = []
@@ -74,25 +82,38 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
     ms, v, cx = min(results)
     engine.set_tiling(v, cx)
     entry = {"variant": v, "cx": cx, "k1_ms": ms, "tile": list(engine.variants()[v])}
-    # FFT passes: lines per block (LDS budget) and block size
+    # FFT passes: lines per block (LDS budget) and block size of the line passes,
+    # then the plane passes (parts per plane and block size) of 3D fields with
+    # 128 x 128 planes, then the line passes again for what the planes leave
     if any(pb.Bhat is not None for pb, _ in engine.slots.values()):
-        fres = []
-        for kb in fft_budgets:
-            for nt in (128, 256, 512):
-                try:
-                    engine.set_fft(kb, nt)
-                    fms = engine.time_kernel("fft", reps=reps)
-                except (ValueError, RuntimeError) as e:
-                    if verbose:
-                        print(f"  fft {kb} KiB / {nt}: skipped ({e})")
-                    continue
-                fres.append((fms, kb, nt))
+        def trial(kb, nt, parts, pnt):
+            try:
+                engine.set_fft(kb, nt, parts, pnt)
+                fms = engine.time_kernel("fft", reps=reps)
+            except (ValueError, RuntimeError) as e:
                 if verbose:
-                    print(f"  fft {kb} KiB / {nt} threads: {fms:.4f} ms")
+                    print(f"  fft {kb} KiB / {nt}, planes {parts} / {pnt}: skipped ({e})")
+                return None
+            if verbose:
+                print(f"  fft {kb} KiB / {nt} threads, planes {parts} / {pnt}: {fms:.4f} ms")
+            return (fms, kb, nt, parts, pnt)
+
+        budgets, nts = ((16,), (128, 256)) if quick else (fft_budgets, (128, 256, 512))
+        plane_cfgs = ((2, 1024), (4, 1024)) if quick else tuple((p, n) for p in (2, 4, 8) for n in (256, 512, 1024))
+        fres = [r for r in (trial(kb, nt, 0, 256) for kb in budgets for nt in nts) if r]
         if fres:
-            fms, kb, nt = min(fres)
-            engine.set_fft(kb, nt)
-            entry.update({"fft_kb": kb, "fft_nt": nt, "fft_ms": fms})
+            best = min(fres)
+            planes = [r for r in (trial(best[1], best[2], parts, pnt) for parts, pnt in plane_cfgs)
+                      if r and engine.launch_info()["fft_parts"]]
+            if planes and min(planes) < best:
+                best = min(planes)
+                if not quick:
+                    again = [r for r in (trial(kb, nt, best[3], best[4]) for kb in (16, 32, 64)
+                                         for nt in (128, 256)) if r]
+                    best = min(again + [best])
+            fms, kb, nt, parts, pnt = best
+            engine.set_fft(kb, nt, parts, pnt)
+            entry.update({"fft_kb": kb, "fft_nt": nt, "fft_parts": parts, "fft_plane_nt": pnt, "fft_ms": fms})
     with _lock:
         cache = _load()
         cache[sig] = entry

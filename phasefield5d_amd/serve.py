@@ -135,7 +135,7 @@ class Server:
                 admitted = True
             if admitted and self.autotune:
                 from . import autotune
-                autotune.tune(self.engine, chunks=(16,), reps=3)
+                autotune.tune(self.engine, chunks=(8, 16), reps=3, quick=True)
             if running:
                 ts = time.perf_counter()
                 self.engine.step(self.poll_steps)
