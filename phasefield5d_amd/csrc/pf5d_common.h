@@ -85,8 +85,13 @@ PF5D_DEV void st4g(T* p, const V4<T>& v) { gst<V4<T>>(p, v); }
 
 PF5D_HD int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// float reciprocal of a positive integer for fdivmod: the hardware estimate
+// (1 ulp) is enough, fdivmod corrects the quotient
+PF5D_DEV float frcp(int n) { return __builtin_amdgcn_rcpf((float)n); }
+
 // q = idx / cols, r = idx % cols for idx < 2^20, cols < 2^11, via a float
-// reciprocal and one correction step (avoids the integer-division sequence)
+// reciprocal (exact or frcp) and one correction step (avoids the
+// integer-division sequence): the quotient estimate is off by at most one
 PF5D_DEV void fdivmod(int idx, int cols, float rcp, int& q, int& r) {
   q = (int)((float)idx * rcp);
   r = idx - q * cols;

@@ -219,7 +219,7 @@ PF5D_DEV void stockham_pass(const cx<T>* __restrict__ src, cx<T>* __restrict__ d
                             int Ns, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ roots) {
   const int nb = n / R;
   const int tot = G * nb;
-  const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
+  const float rnb = frcp(nb), rns = frcp(Ns);
   for (int b = threadIdx.x; b < tot; b += NT) {
     int g, j, q, k;
     fdivmod(b, nb, rnb, g, j);
@@ -443,7 +443,7 @@ PF5D_DEV void fft_item_ct(const FftItem& I, const cx<T>* twl, cx<T>* b0, int tid
   const bool paired = (I.kind == FK_R2C || I.kind == FK_C2R);
   const int GL = paired ? (G + 1) / 2 : G;
   cx<T>* b1 = b0 + GL * LS;
-  const float rgl = 1.0f / (float)GL;
+  const float rgl = frcp(GL);
   // strided lines read and write global memory from the first / last pass
   const bool fuse_in = I.ps_in != 1, fuse_out = I.ps_out != 1;
   auto fetch_global = [&](int g, int m) -> cx<T> { return item_input<T>(I, G, N, NH, g, m); };
@@ -552,7 +552,7 @@ PF5D_DEV void fft_item_rt(const FftItem& I, const LinePlanDev& pl, const cx<T>* 
   const bool paired = (I.kind == FK_R2C || I.kind == FK_C2R);
   const int GL = paired ? (G + 1) / 2 : G;
   cx<T>* buf1 = buf0 + GL * LS;
-  const float rn = 1.0f / (float)n, rgl = 1.0f / (float)GL;
+  const float rn = frcp(n), rgl = frcp(GL);
   const bool pm_in = I.ps_in != 1, pm_out = I.ps_out != 1;
   gather_lds<T, NT>(GL * n, tid, buf0, [&](int idx, int& dst) -> cx<T> {
     int g, m;
@@ -579,7 +579,7 @@ PF5D_DEV void fft_item_rt(const FftItem& I, const LinePlanDev& pl, const cx<T>* 
   }
   if (I.kind == FK_R2C) {
     const int totp = G * nh;
-    const float rnh = 1.0f / (float)nh;
+    const float rnh = frcp(nh);
 #pragma unroll 1
     for (int idx = tid; idx < totp; idx += NT) {
       int g, m;

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(NT) k_fftp_fwd(const FftPlaneItem* __restrict_
       if (r0 + 2 * GB < N) load_batch(r0 + 2 * GB);
       auto fetch = [&](int g, int m) -> cx<T> { return bA[g * LS + lpad(m)]; };
       auto natural = [&](int g, int m, const cx<T>& v, cx<T>* other) { other[g * LS + lpad(m)] = v; };
-      CtRun<T, NT, N, 0, 1>::run(GB, 1.0f / (float)GB, tid, twl, bB, bA, false, fetch, false, natural);
+      CtRun<T, NT, N, 0, 1>::run(GB, frcp(GB), tid, twl, bB, bA, false, fetch, false, natural);
       __syncthreads();
       const cx<T>* res = (fft_npass(N) % 2 == 1) ? bB : bA;
       // Z = FFT(a + i b): A[k] = (Z[k] + conj(Z[-k])) / 2, B[k] = -i (Z[k] - conj(Z[-k])) / 2
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(NT) k_fftp_fwd(const FftPlaneItem* __restrict_
     const int nhp = I.nhp;
     auto fetch_pl = [&](int g, int m) -> cx<T> { return PL[g * LS + lpad(m)]; };
     auto sink_c = [&](int g, int m, const cx<T>& v, cx<T>*) { gst<cx<T>>(C + (long)m * nhp + k0 + g, v); };
-    CtRun<T, NT, N, 0, 1>::run(nk, 1.0f / (float)nk, tid, twl, B0, PL, false, fetch_pl, true, sink_c);
+    CtRun<T, NT, N, 0, 1>::run(nk, frcp(nk), tid, twl, B0, PL, false, fetch_pl, true, sink_c);
     __syncthreads();
   }
 }
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(NT) k_fftp_inv(const FftPlaneItem* __restrict_
       auto sink_pl = [&](int g, int m, const cx<T>& v, cx<T>*) {
         if (m >= y0 && m < y1) PL[(m - y0) * LSR + lpad(c0 + g)] = conjg(v);
       };
-      CtRun<T, NT, N, 0, 1>::run(gc, 1.0f / (float)gc, tid, twl, B0, B1y, true, fetch_c, false, sink_pl);
+      CtRun<T, NT, N, 0, 1>::run(gc, frcp(gc), tid, twl, B0, B1y, true, fetch_c, false, sink_pl);
       __syncthreads();
     }
     // rows stage (C2R) on the kept row pairs: complex line g carries
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(NT) k_fftp_inv(const FftPlaneItem* __restrict_
       return {ca.x + cb.y, ca.y - cb.x};
     };
     auto natural = [&](int g, int m, const cx<T>& v, cx<T>* other) { other[g * LS + lpad(m)] = v; };
-    CtRun<T, NT, N, 0, 1>::run(np, 1.0f / (float)np, tid, twl, B0, B1r, false, fetch_pl, false, natural);
+    CtRun<T, NT, N, 0, 1>::run(np, frcp(np), tid, twl, B0, B1r, false, fetch_pl, false, natural);
     __syncthreads();
     const cx<T>* res = (fft_npass(N) % 2 == 1) ? B0 : B1r;
     // C2R: Re to row y0 + 2g, -Im to row y0 + 2g + 1, point-contiguous

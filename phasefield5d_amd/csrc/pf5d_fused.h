@@ -133,8 +133,8 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
 
     // X planes go global -> registers (issued one iteration ahead) -> LDS
     constexpr int XPT = ((BY + 4) * (BZ + 4) + NT - 1) / NT;
-    const float rx = 1.0f / (float)xcols, rm = 1.0f / (float)(tz + 2), rt = 1.0f / (float)tz,
-                rt1 = 1.0f / (float)(tz + 1);
+    const float rx = frcp(xcols), rm = frcp(tz + 2), rt = frcp(tz),
+                rt1 = frcp(tz + 1);
     V4<T> xr[XPT];
     auto fetch_plane = [&](int p) {
       const int gx = hx ? wrap(p, n0) : 0;

@@ -64,7 +64,7 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
     const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
     const bool hx = sd.dim == 3, hy = sd.dim >= 2;
     int gx, gy, gz;
-    const bool valid = tile_cell<T>(sd, item, tid, 1.0f / (float)item.tz, gx, gy, gz);
+    const bool valid = tile_cell<T>(sd, item, tid, frcp(item.tz), gx, gy, gz);
     const long c = ((long)gx * n1 + gy) * n2 + gz;
     V4<T> xc{};
     if (valid) xc = ld4g<T>(sd.X + 4 * c);
@@ -74,16 +74,29 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
     int lo[4];
     {
       const V4<T> xa = ld4g<T>(sd.X + 4 * (((long)item.xa * n1 + item.y0) * n2 + item.z0));
+      // (the box position only decides which points read LDS: clamped in
+      // floating point first, so NaN lands on 0 and the conversion is in range)
+      const float top = (float)(num - BOX4);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) lo[a] = min(max(floor_index<T>(xa.v[a] * sd.tab.inv_step, num) - 1, 0), num - BOX4);
+      for (int a = 0; a < 4; ++a)
+        lo[a] = (int)fminf(fmaxf((float)(xa.v[a] * sd.tab.inv_step) - 1.0f, 0.0f), top);
     }
-    for (int e = tid; e < NBOX * NCOL; e += NT) {
-      const int node = e / NCOL, col = e - node * NCOL;
-      const long row = ((((long)(lo[0] + (node >> 6)) * num + lo[1] + ((node >> 4) & 3)) * num + lo[2] +
-                        ((node >> 2) & 3)) * num + lo[3] + (node & 3));
-      const int slot = (node >> 6) * L::S0 + ((node >> 4) & 3) * L::S1 + ((node >> 2) & 3) * L::S2 +
-                       (node & 3) * L::S3 + col;
-      reinterpret_cast<VT*>(BOX)[slot] = gld<VT>(sd.tab.data + row * NP + col * Vec16<T>::n);
+    {
+      // one node per thread, all of its columns: the box's first row (64-bit),
+      // then offsets within the box (below 4 num^3 NP, 32-bit)
+      const T* row0 = sd.tab.data + ((((long)lo[0] * num + lo[1]) * num + lo[2]) * num + lo[3]) * NP;
+      const int n2 = num * num, n3 = n2 * num;
+#pragma unroll 1
+      for (int node = tid; node < NBOX; node += NT) {
+        const int q0 = node >> 6, q1 = (node >> 4) & 3, q2 = (node >> 2) & 3, q3 = node & 3;
+        const T* src = row0 + (q0 * n3 + q1 * n2 + q2 * num + q3) * NP;
+        VT* dst = reinterpret_cast<VT*>(BOX) + (q0 * L::S0 + q1 * L::S1 + q2 * L::S2 + q3 * L::S3);
+        VT v[NCOL];
+#pragma unroll
+        for (int col = 0; col < NCOL; ++col) v[col] = gld<VT>(src + col * Vec16<T>::n);
+#pragma unroll
+        for (int col = 0; col < NCOL; ++col) dst[col] = v[col];
+      }
     }
     __syncthreads();
     if (valid) {
@@ -214,7 +227,7 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
     const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
     const bool hx = sd.dim == 3, hy = sd.dim >= 2;
     int gx, gy, gz;
-    const bool valid = tile_cell<T>(sd, item, tid, 1.0f / (float)item.tz, gx, gy, gz);
+    const bool valid = tile_cell<T>(sd, item, tid, frcp(item.tz), gx, gy, gz);
     if constexpr (LOADS < 2) {
       if (!valid) continue;
     } else if (!valid) {
