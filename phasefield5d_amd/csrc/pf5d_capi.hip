@@ -615,8 +615,11 @@ struct K1Variant {
   void (*fflux)(const SimDev<T>*, const SimState*, T*, const K1Item*, int);
   // flux pass of the two-pass form: fcpl == 0 runs on the potentials pass's
   // per-plane tiles; fcpl > 0 marches along axis 0 over tiles of rows of
-  // 64*fcpl cells, fnt/64 rows per block
+  // 64*fcpl cells, fnt/64 rows per block (fmarch, which can also run the dt
+  // controller in its last block)
   int fcpl = 0, fnt = 0;
+  void (*fmarch)(const SimDev<T>*, SimState*, T*, const K1Item*, int, const double*, const int*, int,
+                 unsigned*) = nullptr;
   bool split() const { return by == 0; }
 };
 
@@ -634,7 +637,7 @@ static K1Variant<T> make_split_variant() {
 // two-pass form with the marching flux pass (FNT threads, CPL cells per lane)
 template <typename T, int NT, int MINW, int FNT, int CPL>
 static K1Variant<T> make_march_variant() {
-  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxmarch<T, FNT, CPL>, CPL, FNT};
+  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL>};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -684,6 +687,7 @@ struct Engine : EngineBase {
   SimDev<T>* d_sims = nullptr;
   SimState* d_state = nullptr;
   T* d_xmax = nullptr;
+  unsigned* d_done = nullptr;  // block counter of the flux pass that runs the dt controller
   K1Item* d_k1 = nullptr;
   K1Item* d_kf = nullptr;  // tiles of the marching flux pass
   K2Item* d_k2 = nullptr;
@@ -703,6 +707,8 @@ struct Engine : EngineBase {
       HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       own_stream = true;
     }
+    d_done = (unsigned*)dmalloc(sizeof(unsigned));
+    HIPCHK(hipMemset(d_done, 0, sizeof(unsigned)));
     for (auto& v : variants)
       if (!v.split())
         (void)hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds);
@@ -724,6 +730,7 @@ struct Engine : EngineBase {
   ~Engine() override {
     drop_graphs();
     for (auto& s : sims) free_sim(s);
+    dfree(d_done);
     dfree(d_sims);
     dfree(d_state);
     dfree(d_xmax);
@@ -1006,10 +1013,11 @@ struct Engine : EngineBase {
     int occ = 1;
     if (V.split()) {
       auto grid = [&](int o) { return xcd_grid(std::min(nk1, devinfo().cus * std::max(1, o))); };
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fpot, V.nt, 0);
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fpot, V.nt, 0));
       k1_grid = grid(occ);
       occ = 1;
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fflux, V.fnt, 0);
+      const void* ff = V.fcpl > 0 ? (const void*)V.fmarch : (const void*)V.fflux;
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ff, V.fnt, 0));
       kf_grid = V.fcpl > 0 ? xcd_grid(std::min(nkf, devinfo().cus * std::max(1, occ))) : grid(occ);
     } else {
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
@@ -1034,34 +1042,44 @@ struct Engine : EngineBase {
     HIPCHK(hipGetLastError());
   }
   void launch_dt(int advance) {
-    hipLaunchKernelGGL(k_dtctl<T>, dim3((unsigned)sims.size()), dim3(DT_NT), 0, stream, d_sims, d_state, d_xmax,
+    hipLaunchKernelGGL((k_dtctl<T, DT_NT>), dim3((unsigned)sims.size()), dim3(DT_NT), 0, stream, d_sims, d_state, d_xmax,
                        d_part, d_prange, (int)sims.size(), advance);
     HIPCHK(hipGetLastError());
   }
-  void launch_flux(const K1Variant<T>& V) {
-    if (V.fcpl > 0)
-      hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.fnt), 0, stream, d_sims, d_state, d_xmax, d_kf, nkf);
-    else
-      hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+  // the flux pass; with_dt: the marching pass also runs the dt controller
+  // (returns whether it did)
+  bool launch_flux(const K1Variant<T>& V, bool with_dt = false) {
+    if (V.fcpl > 0) {
+      const int ndt = (with_dt && fuse_dt_ok()) ? (int)sims.size() : 0;
+      hipLaunchKernelGGL(V.fmarch, dim3(kf_grid), dim3(V.fnt), 0, stream, d_sims, d_state, d_xmax, d_kf, nkf, d_part,
+                         d_prange, ndt, d_done);
+      return ndt > 0;
+    }
+    hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+    return false;
   }
-  void launch_k1() {
-    if (!nk1) return;
+  // the flux pass's last block runs the controllers of a few simulations (a
+  // launch of one block per simulation is better for many)
+  bool fuse_dt_ok() const { return sims.size() <= 8 && nk2 > 0; }
+  // K1; with_dt: the dt controller too where the flux pass can run it (returns whether it did)
+  bool launch_k1(bool with_dt = false) {
+    if (!nk1) return false;
     const K1Variant<T>& V = variants[variant];
     if (V.split()) {
       hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
       HIPCHK(hipGetLastError());
-      launch_flux(V);
+      const bool did = launch_flux(V, with_dt);
       HIPCHK(hipGetLastError());
-      return;
+      return did;
     }
     require(V.lds <= (size_t)devinfo().lds, "K1 tile needs more LDS than a block may use");
     hipLaunchKernelGGL(V.fn, dim3(k1_grid), dim3(V.nt), V.lds, stream, d_sims, d_state, d_xmax, d_k1, nk1);
     HIPCHK(hipGetLastError());
+    return false;
   }
   void enqueue_step() {
     fft.run(stream, d_state);
-    launch_k1();
-    launch_dt(1);
+    if (!launch_k1(true)) launch_dt(1);
     launch_k2(0);
   }
   hipGraphExec_t capture(int n) {

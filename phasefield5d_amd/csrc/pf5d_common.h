@@ -104,12 +104,13 @@ PF5D_DEV void fdivmod(int idx, int cols, float rcp, int& q, int& r) {
   }
 }
 
-// non-negative IEEE values order like their bit patterns: max via integer atomics
-PF5D_DEV void atomic_max_nonneg(double* addr, double v) {
-  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+// non-negative IEEE values order like their bit patterns: max via integer
+// atomics (device scope); returns the previous bits
+PF5D_DEV unsigned long long atomic_max_nonneg(double* addr, double v) {
+  return atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
 }
-PF5D_DEV void atomic_max_nonneg(float* addr, float v) {
-  atomicMax(reinterpret_cast<unsigned int*>(addr), (unsigned int)__float_as_uint(v));
+PF5D_DEV unsigned long long atomic_max_nonneg(float* addr, float v) {
+  return atomicMax(reinterpret_cast<unsigned int*>(addr), (unsigned int)__float_as_uint(v));
 }
 
 // max that ignores NaN (the reference abs_max only replaces on v > m)

@@ -395,19 +395,19 @@ PF5D_DEV double adaptive_dt(double dt, double x, double ctld, double t_cfl, doub
   return out;
 }
 
-template <typename T>
-__global__ void k_dtctl(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __restrict__ xmax_acc,
-                        const double* __restrict__ part, const int* __restrict__ part_range, int nsims,
-                        int advance) {
-  const int sim = blockIdx.x;
-  if (sim >= nsims) return;
-  __shared__ double red[16][NSTAT];
-  __shared__ double comp[4];
+// The controller for one simulation, run by one block of NTB threads (red: an
+// LDS array of NTB/64 rows): reduce the FP64 partials of the last update (mass,
+// bounds), take max |dX/dt| (and reset it for the next step), then the bounds
+// check, the end time and the adaptive dt (system.py:90-112).
+template <typename T, int NTB>
+PF5D_DEV void dt_control(int sim, const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
+                         T* __restrict__ xmax_acc, const double* __restrict__ part,
+                         const int* __restrict__ part_range, int advance, double (*red)[NSTAT], int tid) {
   const int p0 = part_range[2 * sim], p1 = part_range[2 * sim + 1];
   // Neumaier sums of the mass partials, min/max of the rest
   double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
   double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
-  for (int i = p0 + threadIdx.x; i < p1; i += blockDim.x) {
+  for (int i = p0 + tid; i < p1; i += NTB) {
     const double* q = part + (long)i * NSTAT;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -431,55 +431,69 @@ __global__ void k_dtctl(const SimDev<T>* __restrict__ sims, SimState* __restrict
     v[7] = fmin(v[7], __shfl_xor(v[7], off, 64));
     v[8] = fmax(v[8], __shfl_xor(v[8], off, 64));
   }
-  if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < NSTAT; ++k) red[threadIdx.x >> 6][k] = v[k];
+  if ((tid & 63) == 0)
+    for (int k = 0; k < NSTAT; ++k) red[tid >> 6][k] = v[k];
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  (void)comp;
-  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-    for (int a = 0; a < 5; ++a) v[a] += red[w][a];
-    v[5] = fmax(v[5], red[w][5]);
-    v[6] = fmin(v[6], red[w][6]);
-    v[7] = fmin(v[7], red[w][7]);
-    v[8] = fmax(v[8], red[w][8]);
-  }
-  SimState& S = st[sim];
-  const SimDev<T>& sd = sims[sim];
-  const double x = (double)xmax_acc[sim];
-  xmax_acc[sim] = T(0);
-  if (!(S.status & ST_ACTIVE)) return;
-  if (p1 > p0) {
-    for (int a = 0; a < 4; ++a) {
-      S.mass[a] = v[a];
-      S.mean[a] = v[a] / (double)sd.ncell;
+  if (tid == 0) {
+    for (int w = 1; w < NTB / 64; ++w) {
+      for (int a = 0; a < 5; ++a) v[a] += red[w][a];
+      v[5] = fmax(v[5], red[w][5]);
+      v[6] = fmin(v[6], red[w][6]);
+      v[7] = fmin(v[7], red[w][7]);
+      v[8] = fmax(v[8], red[w][8]);
     }
-    S.nonfinite = v[4];
-    S.smax = v[5];
-    S.smin = v[6];
-    S.cmin = v[7];
-    S.cmax = v[8];
+    SimState& S = st[sim];
+    const SimDev<T>& sd = sims[sim];
+    // atomic exchange: the maxima were folded in with device-scope atomics
+    const double x = (double)__hip_atomic_exchange(&xmax_acc[sim], T(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    [&] {
+      if (!(S.status & ST_ACTIVE)) return;
+      if (p1 > p0) {
+        for (int a = 0; a < 4; ++a) {
+          S.mass[a] = v[a];
+          S.mean[a] = v[a] / (double)sd.ncell;
+        }
+        S.nonfinite = v[4];
+        S.smax = v[5];
+        S.smin = v[6];
+        S.cmin = v[7];
+        S.cmax = v[8];
+      }
+      if (!advance) return;
+      // bounds check of X_n before step n (the reference checks every 100 steps)
+      if (sd.oob_interval > 0 && (S.steps % sd.oob_interval) == 0) {
+        const double th = sd.tol_hard, ts = sd.tol_soft;
+        const bool hard = S.nonfinite > 0 || S.smax > 1.0 + th || S.smin < -th || S.cmin < -th || S.cmax > 1.0 + th;
+        const bool soft = S.smax > 1.0 + ts || S.smin < -ts || S.cmin < -ts || S.cmax > 1.0 + ts;
+        if (soft) S.status |= ST_SOFT_OOB;
+        if (hard) {
+          S.status = (S.status & ~ST_ACTIVE) | ST_HARD_OOB;
+          return;
+        }
+      }
+      if (S.t >= sd.t_end) {
+        S.status = (S.status & ~ST_ACTIVE) | ST_DONE;
+        return;
+      }
+      const double dt = adaptive_dt(S.dt, x, sd.ctld, sd.dt_cfl, sd.lower, sd.upper);
+      S.dt = dt;
+      S.t = S.t + dt;
+      S.last_xmax = x;
+      S.steps += 1;
+    }();
   }
-  if (!advance) return;
-  // bounds check of X_n before step n (the reference checks every 100 steps)
-  if (sd.oob_interval > 0 && (S.steps % sd.oob_interval) == 0) {
-    const double th = sd.tol_hard, ts = sd.tol_soft;
-    const bool hard = S.nonfinite > 0 || S.smax > 1.0 + th || S.smin < -th || S.cmin < -th || S.cmax > 1.0 + th;
-    const bool soft = S.smax > 1.0 + ts || S.smin < -ts || S.cmin < -ts || S.cmax > 1.0 + ts;
-    if (soft) S.status |= ST_SOFT_OOB;
-    if (hard) {
-      S.status = (S.status & ~ST_ACTIVE) | ST_HARD_OOB;
-      return;
-    }
-  }
-  if (S.t >= sd.t_end) {
-    S.status = (S.status & ~ST_ACTIVE) | ST_DONE;
-    return;
-  }
-  const double dt = adaptive_dt(S.dt, x, sd.ctld, sd.dt_cfl, sd.lower, sd.upper);
-  S.dt = dt;
-  S.t = S.t + dt;
-  S.last_xmax = x;
-  S.steps += 1;
+  __syncthreads();  // red is reused by the next call
+}
+
+// one block per simulation
+template <typename T, int NTB>
+__global__ void __launch_bounds__(NTB) k_dtctl(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
+                                               T* __restrict__ xmax_acc, const double* __restrict__ part,
+                                               const int* __restrict__ part_range, int nsims, int advance) {
+  const int sim = blockIdx.x;
+  if (sim >= nsims) return;
+  __shared__ double red[NTB / 64][NSTAT];
+  dt_control<T, NTB>(sim, sims, st, xmax_acc, part, part_range, advance, red, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -381,10 +381,17 @@ PF5D_DEV CellV<T> shfl_cell_from(const CellV<T>& v, int src) {
 // Faces are computed with the lower cell first, so a face evaluated once gives
 // the same bits as the reference's two evaluations (the face flux is symmetric),
 // and the divergence is summed in the reference order.
+//
+// With ndt > 0 the block that finishes last also runs the dt controller of
+// simulations 0 .. ndt-1 (the step's DT stage, without a launch of its own):
+// every block counts itself in *done after its maxima are folded in; the last
+// one to count resets the counter for the next launch.
 template <typename T, int NT, int CPL>
-__global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st,
+__global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
                                                   T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
-                                                  int nitems) {
+                                                  int nitems, const double* __restrict__ part,
+                                                  const int* __restrict__ part_range, int ndt,
+                                                  unsigned* __restrict__ done) {
   constexpr int NW = NT / 64;
   constexpr int QN = 16 / sizeof(T);
   constexpr int QC = (13 * CPL + QN - 1) / QN;  // 16-byte words per lane: cells
@@ -397,6 +404,7 @@ __global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ 
   block_range(nitems, ib, ie);
   T tmax = T(0);
   int cur = -1;
+  unsigned long long folded = 0;  // previous bits returned by the maxima's atomics
   auto flush = [&](int sim) {
     const T m = wave_max<T>(tmax);
     if (lane == 0) RED[w] = m;
@@ -404,7 +412,7 @@ __global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ 
     if (tid == 0) {
       T r = RED[0];
       for (int v = 1; v < NW; ++v) r = (RED[v] > r) ? RED[v] : r;
-      atomic_max_nonneg(&xmax_acc[sim], r);
+      folded |= atomic_max_nonneg(&xmax_acc[sim], r);
     }
     __syncthreads();
     tmax = T(0);
@@ -593,6 +601,22 @@ __global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ 
     }
   }
   if (cur >= 0) flush(cur);
+  if (ndt > 0) {
+    __shared__ int last;
+    __shared__ double DRED[NW][NSTAT];
+    if (tid == 0) {
+      // the count depends on the values the maxima's atomics returned, so it is
+      // made after they were performed (at device scope, where the controller's
+      // exchange reads them); no cache write-back is needed for that
+      const unsigned one = 1u + (folded == ~0ull ? 1u : 0u);  // the bits of a NaN with the sign set: never stored
+      last = atomicAdd(done, one) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      for (int s = 0; s < ndt; ++s) dt_control<T, NT>(s, sims, st, xmax_acc, part, part_range, 1, DRED, tid);
+      if (tid == 0) atomicExch(done, 0u);
+    }
+  }
 }
 
 }  // namespace pf5d
