@@ -34,8 +34,8 @@ namespace pf5d {
 #define PF5D_POTENTIALS_INSTANCE(T, NT, MINW)                                                              \
   PF5D_INST __global__ void k_potentials<T, NT, MINW>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
                                                       const K1Item* __restrict__, int);
-#define PF5D_FLUXM_INSTANCE(T, NT, CPL)                                                                    \
-  PF5D_INST __global__ void k_fluxmarch<T, NT, CPL>(const SimDev<T>* __restrict__, SimState* __restrict__, \
+#define PF5D_FLUXM_INSTANCE(T, NT, CPL, MINW)                                                              \
+  PF5D_INST __global__ void k_fluxmarch<T, NT, CPL, MINW>(const SimDev<T>* __restrict__, SimState* __restrict__, \
                                                     T* __restrict__, const K1Item* __restrict__, int,       \
                                                     const double* __restrict__, const int* __restrict__, int, \
                                                     unsigned* __restrict__);
@@ -90,9 +90,11 @@ namespace pf5d {
   PF5D_FLUX_INSTANCE(float, 256, 0)               \
   PF5D_FLUX_INSTANCE(float, 256, 1)               \
   PF5D_FLUX_INSTANCE(float, 256, 2)               \
-  PF5D_FLUXM_INSTANCE(double, 256, 1)             \
-  PF5D_FLUXM_INSTANCE(float, 256, 1)              \
-  PF5D_FLUXM_INSTANCE(float, 256, 2)              \
-  PF5D_FLUXM_INSTANCE(float, 512, 2)
+  PF5D_FLUXM_INSTANCE(double, 256, 1, 1)          \
+  PF5D_FLUXM_INSTANCE(double, 256, 1, 3)          \
+  PF5D_FLUXM_INSTANCE(float, 256, 1, 1)           \
+  PF5D_FLUXM_INSTANCE(float, 256, 2, 1)           \
+  PF5D_FLUXM_INSTANCE(float, 256, 2, 3)           \
+  PF5D_FLUXM_INSTANCE(float, 512, 2, 1)
 
 }  // namespace pf5d

@@ -386,8 +386,9 @@ PF5D_DEV CellV<T> shfl_cell_from(const CellV<T>& v, int src) {
 // simulations 0 .. ndt-1 (the step's DT stage, without a launch of its own):
 // every block counts itself in *done after its maxima are folded in; the last
 // one to count resets the counter for the next launch.
-template <typename T, int NT, int CPL>
-__global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
+// MINW: minimum waves per SIMD the register allocation must allow
+template <typename T, int NT, int CPL, int MINW>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8))) k_fluxmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
                                                   T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
                                                   int nitems, const double* __restrict__ part,
                                                   const int* __restrict__ part_range, int ndt,
@@ -515,47 +516,39 @@ __global__ void __launch_bounds__(NT) k_fluxmarch(const SimDev<T>* __restrict__ 
       // z: the face above each cell, then the face below from the cell before
       T fzu[CPL][4], fzd[CPL][4];
       {
-        const CellV<T> nb = shfl_cell_from<T>(me[0], (lane + 1) & 63);  // the next lane's first cell
-        CellV<T> e;  // the cell above the row's last cell
-        if (zwrap) {
-          e = shfl_cell_from<T>(me[0], 0);
-        } else {
-          e = nb;
-          if (lane == lend) e = load_cell<T>(sd, row_base(x, gy) + (item.z0 + tz < n2 ? item.z0 + tz : 0));
-        }
+        // the cell after the lane's last one: the next lane's first, or, for the
+        // row's last cell, the row's first (periodic tile) or the cell after the tile
+        const bool ends = lane == lend;
+        CellV<T> nb = shfl_cell_from<T>(me[0], (zwrap && ends) ? 0 : ((lane + 1) & 63));
+        if (!zwrap && ends) nb = load_cell<T>(sd, row_base(x, gy) + (item.z0 + tz < n2 ? item.z0 + tz : 0));
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
-          const bool last = lane == lend && j == jend;
+          const bool last = ends && j == jend;
           const CellV<T>& nxt = (j + 1 < CPL) ? me[j + 1 < CPL ? j + 1 : 0] : nb;
           CellV<T> u;
-          u.X = last ? e.X : nxt.X;
-          u.mu = last ? e.mu : nxt.mu;
+          u.X = last ? nb.X : nxt.X;
+          u.mu = last ? nb.mu : nxt.mu;
 #pragma unroll
-          for (int m = 0; m < 5; ++m) u.M[m] = last ? e.M[m] : nxt.M[m];
+          for (int m = 0; m < 5; ++m) u.M[m] = last ? nb.M[m] : nxt.M[m];
           face_ab<T>(me[j], u, inv_dx, fzu[j]);
         }
 #pragma unroll
         for (int j = 1; j < CPL; ++j)
 #pragma unroll
           for (int a = 0; a < 4; ++a) fzd[j][a] = fzu[j - 1][a];
-        // the previous lane's last face; lane 0: the row's last face (periodic) or its own
-        T wl[4];
+        // below the lane's first cell: the previous lane's last face; lane 0 of a
+        // periodic tile: the row's last face (the lane holding the row's last
+        // cell offers that one, its later slots being outside the row)
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-          T v = fzu[0][a];
+          T v = fzu[CPL - 1][a];
 #pragma unroll
-          for (int j = 1; j < CPL; ++j) v = (jend == j) ? fzu[j][a] : v;
-          wl[a] = __shfl(v, lend, 64);
-          fzd[0][a] = __shfl(fzu[CPL - 1][a], (lane + 63) & 63, 64);
+          for (int j = 0; j + 1 < CPL; ++j) v = (ends && jend == j) ? fzu[j][a] : v;
+          fzd[0][a] = __shfl(v, (zwrap && lane == 0) ? lend : ((lane + 63) & 63), 64);
         }
-        if (lane == 0) {
-          if (zwrap) {
-#pragma unroll
-            for (int a = 0; a < 4; ++a) fzd[0][a] = wl[a];
-          } else {
-            const CellV<T> dn = load_cell<T>(sd, row_base(x, gy) + (item.z0 > 0 ? item.z0 - 1 : n2 - 1));
-            face_ab<T>(dn, me[0], inv_dx, fzd[0]);
-          }
+        if (lane == 0 && !zwrap) {
+          const CellV<T> dn = load_cell<T>(sd, row_base(x, gy) + (item.z0 > 0 ? item.z0 - 1 : n2 - 1));
+          face_ab<T>(dn, me[0], inv_dx, fzd[0]);
         }
       }
       T fxu[CPL][4];
