@@ -16,3 +16,7 @@ from .elastic import (  # noqa: F401
 from .interpolation import build_4d_grid, build_calphad_kdtree, interpolator_nb  # noqa: F401
 from .operators import calculate_laplacian, calculate_gradients_pm, abs_max, clamp_simplex  # noqa: F401
 from .fluxes import get_divergence_from_mu_function, get_flux_function  # noqa: F401
+from .io import (  # noqa: F401
+    SaveSchedule, make_path, save_current_state, init_traces, update_traces, finalize_traces, get_model_tag,
+)
+from .post_process import iter_npz_states, parse_run_name, npz_to_vtk, npz_to_vti, batch_npz_to_vti  # noqa: F401

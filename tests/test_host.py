@@ -171,3 +171,64 @@ def test_tsv_roundtrip(tmp_path):
     assert np.allclose(df.to_numpy(), tab.rows, rtol=4e-16, atol=0)
     with pytest.raises(FileNotFoundError):
         load_calphad_dataframe("1000K", path=str(tmp_path))
+
+
+def _decode_vti_arrays(text):
+    """name -> float32 values of every binary DataArray of a VTK XML file."""
+    import base64
+    import struct
+    out = {}
+    for name, payload in re.findall(r'Name="(\w+)"[^>]*format="binary">\s*([A-Za-z0-9+/=]+)\s*</DataArray>', text):
+        raw = base64.b64decode(payload)
+        (n,) = struct.unpack("<I", raw[:4])
+        assert n == len(raw) - 4
+        out[name] = np.frombuffer(raw[4:], dtype="<f4")
+    return out
+
+
+def test_post_process_exports_round_trip(tmp_path):
+    """A saved state converts to VTI and legacy VTK whose arrays, read back, are the
+    composition with the dependent component first, x varying fastest; the run
+    directory name parses back; states iterate in step order."""
+    from phasefield5d_amd.solver import io, post_process as pp
+    rng = np.random.default_rng(3)
+    X = rng.uniform(0.0, 0.2, size=(3, 4, 5, 4))
+    data = tmp_path / "data"
+    data.mkdir()
+    io.save_current_state(str(tmp_path), 7, 0.5, 3.5, X)
+    io.save_current_state(str(tmp_path), 2, 0.25, 1.0, X * 0.5)
+    states = list(pp.iter_npz_states(str(data)))
+    assert [s[0] for s in states] == [2, 7] and states[1][1:3] == (3.5, 0.5)
+    assert np.array_equal(states[1][3], X)
+    full = np.concatenate([1.0 - X.sum(axis=-1, keepdims=True), X], axis=-1)
+    # VTI: x fastest means index ((k * Ny) + j) * Nx + i holds point (i, j, k)
+    vti = pp.npz_to_vti(data / "step_000000007.npz", spacing=(2.0, 1.0, 0.5))
+    arrs = _decode_vti_arrays(open(vti).read())
+    assert list(arrs) == ["Fe", "Mn", "Ni", "Co", "Cu"]
+    for c, name in enumerate(arrs):
+        v = arrs[name].reshape(5, 4, 3)
+        for (i, j, k) in [(0, 0, 0), (2, 1, 3), (1, 3, 4)]:
+            assert v[k, j, i] == np.float32(full[i, j, k, c])
+    assert 'WholeExtent="0 2 0 3 0 4"' in open(vti).read() and 'Spacing="2.0 1.0 0.5"' in open(vti).read()
+    written = pp.batch_npz_to_vti(data, tmp_path / "vti", verbose=False)
+    assert [os.path.basename(w) for w in written] == ["step_000000002.vti", "step_000000007.vti"]
+    # legacy VTK needs a file whose single (or "composition") array is the field
+    np.savez(tmp_path / "field.npz", composition=X, other=np.zeros(1))
+    vtk = pp.npz_to_vtk(tmp_path / "field.npz")
+    lines = open(vtk).read().splitlines()
+    assert lines[4] == "DIMENSIONS 3 4 5" and lines[7] == "POINT_DATA 60"
+    vals = []
+    for line in lines[8:]:
+        if line.startswith(("SCALARS", "LOOKUP_TABLE")):
+            continue
+        vals += [float(t) for t in line.split()]
+        assert len(line.split()) <= 9
+    vals = np.array(vals).reshape(5, 5, 4, 3)
+    assert np.allclose(vals, np.transpose(full, (3, 2, 1, 0)).astype(np.float32), rtol=1e-7, atol=0)
+    with pytest.raises(ValueError):
+        pp.npz_to_vtk(data / "step_000000007.npz")  # first array is the (scalar) timestep
+    # run-directory names as make_path writes them (1D: a single cell count)
+    name = "cells160_dx1e-09_fluc1e-03_kappa5e-16_steps10000_dt1e-01s_20260101_120000"
+    r = pp.parse_run_name(name)
+    assert r["cells"] == 160 and r["dx"] == 1e-9 and r["steps"] == 10000 and r["dt"] == 0.1 and r["time"] == "120000"
+    assert pp.parse_run_name("cells128x128_dx1e-09_fluc1e-03") is None
