@@ -257,15 +257,13 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
     auto face = [&](int xa_s, int ma_s, int ja, int ka, int xb_s, int mb_s, int jb, int kb, T* out) {
       const V4<T> Xa = XAT(xa_s, ja, ka), Xb = XAT(xb_s, jb, kb);
       const V4<T> ua = MUAT(ma_s, ja, ka), ub = MUAT(mb_s, jb, kb);
-      T Ma[5], Mb[5], g[4];
+      T Ma[5], Mb[5];
 #pragma unroll
       for (int m = 0; m < 5; ++m) {
         Ma[m] = MBAT(ma_s, m, ja, ka);
         Mb[m] = MBAT(mb_s, m, jb, kb);
       }
-#pragma unroll
-      for (int a = 0; a < 4; ++a) g[a] = (ub.v[a] - ua.v[a]) * inv_dx;
-      face_flux<T>(Xa.v, Xb.v, Ma, Mb, g, out);
+      step_face_flux<T>(Xa.v, Xb.v, Ma, Mb, ua.v, ub.v, inv_dx, out);
     };
 
     const int pbeg = hx ? item.xa - 1 : 0, pend = hx ? item.xb : 0;
@@ -340,7 +338,7 @@ k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __re
             V4<T> o;
 #pragma unroll
             for (int a = 0; a < 4; ++a) {
-              o.v[a] = acc[a] * inv_dx;
+              o.v[a] = acc[a] * step_div_scale<T>(inv_dx);
               tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
             }
             const int gy = hy ? item.y0 + j : 0;

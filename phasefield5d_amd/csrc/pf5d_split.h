@@ -168,10 +168,7 @@ PF5D_DEV CellV<T> load_cell(const SimDev<T>& sd, long c) {
 // flux through the face from cell a (lower) to cell b (upper)
 template <typename T>
 PF5D_DEV void face_ab(const CellV<T>& a, const CellV<T>& b, T inv_dx, T out[4]) {
-  T g[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) g[k] = (b.mu.v[k] - a.mu.v[k]) * inv_dx;
-  face_flux<T>(a.X.v, b.X.v, a.M, b.M, g, out);
+  step_face_flux<T>(a.X.v, b.X.v, a.M, b.M, a.mu.v, b.mu.v, inv_dx, out);
 }
 
 // the X, mu and Mob of lane (lane - 1) (dir = -1) or (lane + 1) (dir = +1) of the wavefront
@@ -236,7 +233,7 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
       gz = item.z0;
     }
     const long c = ((long)gx * n1 + gy) * n2 + gz;
-    const T inv_dx = sd.inv_dx;
+    const T inv_dx = sd.inv_dx, dscale = step_div_scale<T>(inv_dx);
     const long sx = (long)n1 * n2;
     const long ixu = hx ? c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx) : c;
     const long ixd = hx ? c + (gx > 0 ? -sx : (long)(n0 - 1) * sx) : c;
@@ -284,7 +281,7 @@ __global__ void __launch_bounds__(NT) k_fluxdiv(const SimDev<T>* __restrict__ si
     axis(4, izu, izd);
     V4<T> o;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) o.v[a] = acc[a] * inv_dx;
+    for (int a = 0; a < 4; ++a) o.v[a] = acc[a] * dscale;
     if (valid) {
 #pragma unroll
       for (int a = 0; a < 4; ++a) tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
@@ -428,7 +425,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
     const SimDev<T>& sd = sims[item.sim];
     const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
     const bool hx = sd.dim == 3, hy = sd.dim >= 2;
-    const T inv_dx = sd.inv_dx;
+    const T inv_dx = sd.inv_dx, dscale = step_div_scale<T>(inv_dx);
     // wavefronts past the tile's rows work on its first row and store nothing
     const bool row_on = w < item.ty;
     const int gy = item.y0 + (row_on ? w : 0);
@@ -576,7 +573,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
         }
         V4<T> o;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) o.v[a] = acc[a] * inv_dx;
+        for (int a = 0; a < 4; ++a) o.v[a] = acc[a] * dscale;
         if (kon[j]) {
 #pragma unroll
           for (int a = 0; a < 4; ++a) tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
