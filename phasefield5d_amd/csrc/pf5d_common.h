@@ -71,6 +71,26 @@ PF5D_DEV V sld(const void* p) {
   __builtin_memcpy(&v, &w, sizeof(V));
   return v;
 }
+// streaming store: for outputs the next kernels read from other XCDs, so
+// they need not stay dirty in this XCD's L2
+template <typename V>
+PF5D_DEV void gst_nt(void* p, const V& v) {
+  using W = typename Words<sizeof(V)>::type;
+  W w;
+  __builtin_memcpy(&w, &v, sizeof(V));
+  __builtin_nontemporal_store(w, (PF5D_GLOBAL W*)p);
+}
+// store of a stencil-stage output field: streaming in FP64, where an XCD's
+// share of the field is about twice its L2 and nothing of it survives to the
+// next kernel; a plain store in FP32, where the next kernel (same XCD, same
+// slab) still finds part of it in L2
+template <typename T, typename V>
+PF5D_DEV void gst_field(void* p, const V& v) {
+  if constexpr (sizeof(T) == 8)
+    gst_nt<V>(p, v);
+  else
+    gst<V>(p, v);
+}
 template <typename V>
 PF5D_DEV void sst(void* p, const V& v) {
   using W = typename Words<sizeof(V)>::type;

@@ -138,9 +138,9 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
         mu.v[a] = m;
         mb.v[a] = P[4 + a];
       }
-      st4g<T>(sd.mu + 4 * c, mu);
-      st4g<T>(sd.mb + 4 * c, mb);
-      gst<T>(sd.mb1 + c, P[8]);
+      gst_field<T, V4<T>>(sd.mu + 4 * c, mu);
+      gst_field<T, V4<T>>(sd.mb + 4 * c, mb);
+      gst_field<T, T>(sd.mb1 + c, P[8]);
     }
     __syncthreads();  // BOX is reused by the next item
   }
@@ -577,7 +577,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
         if (kon[j]) {
 #pragma unroll
           for (int a = 0; a < 4; ++a) tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
-          st4g<T>(sd.dX + 4 * (r + kz[j]), o);
+          gst_field<T, V4<T>>(sd.dX + 4 * (r + kz[j]), o);
         }
       }
       if (hx) {
