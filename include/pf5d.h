@@ -144,7 +144,8 @@ int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx);
 /* FFT passes: LDS bytes per work item in KiB (sets how many lines one block
  * transforms) and threads per block (128 or 256) */
 int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt);
-/* FFT plane passes for 3D fields whose y and z lengths are a plane length (128):
+/* FFT plane passes for 3D fields whose y and z lengths are a plane length (128, and 64 and 96 with 512 or
+ * 1024 threads and 2 or 4 parts):
  * the z and y transforms of an x-plane in one work item, the plane split into
  * parts (2, 4 or 8) work items of nt threads (256, 512 or 1024); parts = 0: line
  * passes only (the default).  Where the plane passes need more LDS than a block

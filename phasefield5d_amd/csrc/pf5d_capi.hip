@@ -274,6 +274,12 @@ static PlaneKernels<T> plane_kernels_nt(int n, int parts) {
   if (n == 128 && parts == 2) return plane_kernels_np<T, NT, 128, 2>();
   if (n == 128 && parts == 4) return plane_kernels_np<T, NT, 128, 4>();
   if (n == 128 && parts == 8) return plane_kernels_np<T, NT, 128, 8>();
+  if constexpr (NT != 256) {
+    if (n == 64 && parts == 2) return plane_kernels_np<T, NT, 64, 2>();
+    if (n == 64 && parts == 4) return plane_kernels_np<T, NT, 64, 4>();
+    if (n == 96 && parts == 2) return plane_kernels_np<T, NT, 96, 2>();
+    if (n == 96 && parts == 4) return plane_kernels_np<T, NT, 96, 4>();
+  }
   return {};
 }
 // the block sizes, plane lengths and part counts built; fwd == nullptr when there is none
@@ -301,9 +307,15 @@ struct FftBatch {
   // plane passes (3D fields whose y and z lengths are a plane length): parts
   // per plane, 0 = line passes only
   int parts = 0, pnt = 256;  // parts per plane (0 = line passes only), threads per block
-  std::vector<FftPlaneItem> pfwd, pinv;
+  // plane items per plane length (one launch per length and direction)
+  struct PlaneGroup {
+    PlaneKernels<T> k;
+    std::vector<FftPlaneItem> fwd, inv;
+    int off = 0;  // first item in d_pitems (forward items, then inverse items)
+  };
+  std::map<int, PlaneGroup> pgroups;
   FftPlaneItem* d_pitems = nullptr;
-  PlaneKernels<T> pk;
+  bool has_planes() const { return !pgroups.empty(); }
 
   static bool paired(int kind) { return kind == FK_R2C || kind == FK_C2R; }
   // complex lines per item: two line buffers within the LDS budget (real-line
@@ -404,12 +416,13 @@ struct FftBatch {
       // rows and y lines of each plane in the plane passes; the row stages above are dropped
       stage[0].resize(stage[0].size() - nrows_r2c);
       stage[4].resize(stage[4].size() - nrows_c2r);
-      pk = k;
+      PlaneGroup& pg = pgroups[nl];
+      pg.k = k;
       const int plan = plans.get(nl);
       for (int x = 0; x < nx; ++x)
         for (int p = 0; p < parts; ++p) {
-          pfwd.push_back({sim, plan, x, p, s, C, (int)nhp});
-          pinv.push_back({sim, plan, x, p, C, psi, (int)nhp});
+          pg.fwd.push_back({sim, plan, x, p, s, C, (int)nhp});
+          pg.inv.push_back({sim, plan, x, p, C, psi, (int)nhp});
         }
     } else {
     // y passes: for each x plane, lines kz in [0, nh) with point stride nhp
@@ -443,18 +456,20 @@ struct FftBatch {
   void clear() {
     for (auto& s : stage) s.clear();
     for (auto& l : lds) l = 0;
-    pfwd.clear();
-    pinv.clear();
-    pk = {};
+    pgroups.clear();
   }
   void upload() {
     dfree(d_items);
     d_items = nullptr;
     dfree(d_pitems);
     d_pitems = nullptr;
-    if (!pfwd.empty()) {
-      std::vector<FftPlaneItem> pall(pfwd);
-      pall.insert(pall.end(), pinv.begin(), pinv.end());
+    if (has_planes()) {
+      std::vector<FftPlaneItem> pall;
+      for (auto& kv : pgroups) {
+        kv.second.off = (int)pall.size();
+        pall.insert(pall.end(), kv.second.fwd.begin(), kv.second.fwd.end());
+        pall.insert(pall.end(), kv.second.inv.begin(), kv.second.inv.end());
+      }
       d_pitems = (FftPlaneItem*)dmalloc(pall.size() * sizeof(FftPlaneItem));
       HIPCHK(hipMemcpy(d_pitems, pall.data(), pall.size() * sizeof(FftPlaneItem), hipMemcpyHostToDevice));
     }
@@ -469,13 +484,16 @@ struct FftBatch {
     HIPCHK(hipMemcpy(d_items, all.data(), all.size() * sizeof(FftItem), hipMemcpyHostToDevice));
   }
   void run_planes(hipStream_t stream, const SimState* st, bool inverse) {
-    const int n = (int)pfwd.size();
-    if (n == 0) return;
     const DevInfo& di = devinfo();
-    const unsigned grid = xcd_grid(std::min<long>(n, (long)di.cus * 8));
-    hipLaunchKernelGGL(inverse ? pk.inv : pk.fwd, dim3(grid), dim3(pk.nt), inverse ? pk.inv_lds : pk.fwd_lds, stream,
-                       d_pitems + (inverse ? n : 0), n, plans.d_plans, plans.d_tw, st);
-    HIPCHK(hipGetLastError());
+    for (auto& kv : pgroups) {
+      const PlaneGroup& pg = kv.second;
+      const int n = (int)pg.fwd.size();
+      const unsigned grid = xcd_grid(std::min<long>(n, (long)di.cus * 8));
+      hipLaunchKernelGGL(inverse ? pg.k.inv : pg.k.fwd, dim3(grid), dim3(pg.k.nt),
+                         inverse ? pg.k.inv_lds : pg.k.fwd_lds, stream, d_pitems + pg.off + (inverse ? n : 0), n,
+                         plans.d_plans, plans.d_tw, st);
+      HIPCHK(hipGetLastError());
+    }
   }
   void run(hipStream_t stream, const SimState* st) {
     const DevInfo& di = devinfo();
@@ -723,12 +741,14 @@ struct Engine : EngineBase {
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 128>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
-    for (int pnt : {256, 512, 1024})
-      for (int parts : {2, 4, 8}) {
-        const PlaneKernels<T> k = plane_kernels<T>(128, parts, pnt);
-        (void)hipFuncSetAttribute((const void*)k.fwd, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
-        (void)hipFuncSetAttribute((const void*)k.inv, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
-      }
+    for (int pn : {64, 96, 128})
+      for (int pnt : {256, 512, 1024})
+        for (int parts : {2, 4, 8}) {
+          const PlaneKernels<T> k = plane_kernels<T>(pn, parts, pnt);
+          if (!k.fwd) continue;
+          (void)hipFuncSetAttribute((const void*)k.fwd, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+          (void)hipFuncSetAttribute((const void*)k.inv, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+        }
   }
   ~Engine() override {
     drop_graphs();
@@ -1257,7 +1277,7 @@ struct Engine : EngineBase {
     out[7] = (long)fft.budget;
     out[8] = fft.nt;
     out[9] = kf_grid;
-    out[10] = fft.pfwd.empty() ? 0 : fft.parts;
+    out[10] = fft.has_planes() ? fft.parts : 0;
     out[11] = fft.pnt;
   }
 };

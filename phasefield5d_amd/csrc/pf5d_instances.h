@@ -54,8 +54,9 @@ namespace pf5d {
   PF5D_FFT_INSTANCE(double, 256)  \
   PF5D_FFT_INSTANCE(double, 512)
 
-// plane passes: block sizes, lengths and part counts (the FP64 inverse with 2
-// parts needs more LDS than a block has; the host then runs the line passes)
+// plane passes: block sizes, plane lengths (128; 64 and 96 with fewer
+// settings) and part counts (the FP64 inverse of 128 with 2 parts needs more LDS
+// than a block has; the host then runs the line passes)
 #define PF5D_FFTP_INSTANCES_T(T)     \
   PF5D_FFTP_INSTANCE(T, 256, 128, 2)  \
   PF5D_FFTP_INSTANCE(T, 256, 128, 4)  \
@@ -65,7 +66,15 @@ namespace pf5d {
   PF5D_FFTP_INSTANCE(T, 512, 128, 8)  \
   PF5D_FFTP_INSTANCE(T, 1024, 128, 2) \
   PF5D_FFTP_INSTANCE(T, 1024, 128, 4) \
-  PF5D_FFTP_INSTANCE(T, 1024, 128, 8)
+  PF5D_FFTP_INSTANCE(T, 1024, 128, 8) \
+  PF5D_FFTP_INSTANCE(T, 512, 64, 2)   \
+  PF5D_FFTP_INSTANCE(T, 512, 64, 4)   \
+  PF5D_FFTP_INSTANCE(T, 1024, 64, 2)  \
+  PF5D_FFTP_INSTANCE(T, 1024, 64, 4)  \
+  PF5D_FFTP_INSTANCE(T, 512, 96, 2)   \
+  PF5D_FFTP_INSTANCE(T, 512, 96, 4)   \
+  PF5D_FFTP_INSTANCE(T, 1024, 96, 2)  \
+  PF5D_FFTP_INSTANCE(T, 1024, 96, 4)
 #define PF5D_FFTP_INSTANCES  \
   PF5D_FFTP_INSTANCES_T(float) \
   PF5D_FFTP_INSTANCES_T(double)

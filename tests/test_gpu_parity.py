@@ -335,6 +335,28 @@ def test_fft_plane_passes_bitwise(dev, precision):
             assert np.array_equal(a, b), f"parts {parts}"
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fft_plane_passes_mixed_lengths_bitwise(dev, precision):
+    """One engine holding 64^3, 96^3 (three-pass plan) and 128^3 fields and a 2D
+    field: the plane passes, one launch per plane length, give the same bits as
+    the line passes for every simulation."""
+    from phasefield5d_amd.engine import Engine
+    pbs = [problem(3, 64, seed=11), problem(3, 96, seed=12), problem(3, 128, seed=13), problem(2, 64, seed=14)]
+    for p in pbs:
+        p.X_init = developed_state(p, amp=0.03)
+    out = {}
+    for parts in (0, 4):
+        e = Engine(precision)
+        slots = [e.add_problem(p) for p in pbs]
+        e.set_fft(16, 128, parts, 512)
+        e.step(2)
+        out[parts] = [(e.field(s, "X"), e.field(s, "psi")) for s in slots]
+        assert e.launch_info()["fft_parts"] == parts
+        e.close()
+    for (Xa, pa), (Xb, pb_) in zip(out[0], out[4]):
+        assert np.array_equal(Xa, Xb) and np.array_equal(pa, pb_)
+
+
 def test_batched_equals_alone(dev):
     """Each simulation's result is bit-identical alone and inside a mixed batch."""
     from phasefield5d_amd.engine import Engine
