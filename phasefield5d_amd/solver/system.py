@@ -52,16 +52,22 @@ def out_of_bounds(composition, tol_soft=1e-5, tol_hard=1e-3):
 def apply_fluctuations(reference_composition, size, fluctuation, seed=None):
     """X0 + f * (uniform(-1,1) noise over 5 components, per-cell mean removed)[..., 1:] (system.py:56-61).
 
-    ``seed`` (added) seeds NumPy's legacy global generator before the identical draw.
+    ``seed`` (added) draws from a legacy MT19937 generator seeded with it: the same
+    sequence as seeding NumPy's global generator first, without touching the global
+    state, so concurrent set-ups (the server's worker threads) stay reproducible.
+    Without a seed the global generator is used, as in the reference.  The
+    arithmetic is the reference's, done in place.
     """
-    if seed is not None:
-        np.random.seed(seed)
+    rs = np.random.RandomState(seed) if seed is not None else np.random
     ref = np.asarray(reference_composition, dtype=np.float64)
     n = ref.shape[-1]
-    base = np.broadcast_to(ref, (*size, n))
-    noise = (np.random.rand(*size, n + 1) - 0.5) * 2.0
+    noise = rs.rand(*size, n + 1)
+    noise -= 0.5
+    noise *= 2.0
     noise -= noise.mean(axis=-1, keepdims=True)
-    return base + fluctuation * noise[..., 1:]
+    out = noise[..., 1:] * fluctuation
+    out += ref
+    return out
 
 
 def get_time_increment(ctld, steps_per_ctld=100):
