@@ -357,6 +357,39 @@ def test_fft_plane_passes_mixed_lengths_bitwise(dev, precision):
         assert np.array_equal(Xa, Xb) and np.array_equal(pa, pb_)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_stencil_forms_bitwise_uneven_shapes(dev, precision):
+    """Every stencil tiling gives the same bits on shapes that exercise the tile
+    edges: no more rows than a block has wavefronts (the periodic y wrap inside a
+    tile; 4 cells is the engine's minimum per axis), a row just longer than a
+    tile (a 2-cell tail), odd row lengths (a lane
+    holding the row's last cell in its first slot), a short 2D plane, a 1D line
+    of odd length (elasticity off: the stencil alone)."""
+    import dataclasses
+    from phasefield5d_amd.engine import Engine
+    base = {3: problem(3, 8, seed=21, elastic=False), 2: problem(2, 8, seed=22, elastic=False),
+            1: problem(1, 8, seed=23, elastic=False)}
+    pbs = []
+    for shape in [(6, 4, 130), (5, 4, 33), (4, 9, 65), (7, 200), (129,)]:
+        p = dataclasses.replace(base[len(shape)], shape=tuple(shape), Bhat=None)
+        p.X_init = developed_state(p, amp=0.03, seed=len(pbs) + 3)
+        pbs.append(p)
+    probe = Engine(precision)
+    nvar = len(probe.variants())
+    probe.close()
+    out = {}
+    for v in range(nvar):
+        e = Engine(precision)
+        slots = [e.add_problem(p) for p in pbs]
+        e.set_tiling(v, 2)
+        e.step(2)
+        out[v] = [(e.field(s, "X"), e.field(s, "dXdt"), e.state(s)["dt"]) for s in slots]
+        e.close()
+    for v in range(1, nvar):
+        for i, ((Xa, da, ta), (Xb, db, tb)) in enumerate(zip(out[0], out[v])):
+            assert np.array_equal(Xa, Xb) and np.array_equal(da, db) and ta == tb, f"variant {v} shape {pbs[i].shape}"
+
+
 def test_batched_equals_alone(dev):
     """Each simulation's result is bit-identical alone and inside a mixed batch."""
     from phasefield5d_amd.engine import Engine
