@@ -654,9 +654,9 @@ static K1Variant<T> make_split_variant() {
 }
 // two-pass form with the marching flux pass (FNT threads, CPL cells per lane,
 // FMINW waves per SIMD at least)
-template <typename T, int NT, int MINW, int FNT, int CPL, int FMINW = 1>
+template <typename T, int NT, int MINW, int FNT, int CPL, int FMINW = 1, int BZ = 64>
 static K1Variant<T> make_march_variant() {
-  return {0, 64, NT, 0, nullptr, &k_potentials<T, NT, MINW>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL, FMINW>};
+  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL, FMINW>};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -668,7 +668,7 @@ std::vector<K1Variant<double>> k1_variants<double>() {
           make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, 0>(),
           make_split_variant<double, 256, 2, 1>(), make_split_variant<double, 256, 2, 1, 32>(),
           make_split_variant<double, 256, 2, 2>(), make_march_variant<double, 256, 2, 256, 1>(),
-          make_march_variant<double, 256, 2, 256, 1, 3>()};
+          make_march_variant<double, 256, 2, 256, 1, 3>(), make_march_variant<double, 256, 2, 256, 1, 3, 16>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
@@ -679,7 +679,8 @@ std::vector<K1Variant<float>> k1_variants<float>() {
           make_split_variant<float, 256, 4, 1>(), make_split_variant<float, 256, 4, 1, 32>(),
           make_split_variant<float, 256, 4, 2>(), make_march_variant<float, 256, 4, 256, 1>(),
           make_march_variant<float, 256, 4, 256, 2>(), make_march_variant<float, 256, 4, 512, 2>(),
-          make_march_variant<float, 256, 4, 256, 2, 3>()};
+          make_march_variant<float, 256, 4, 256, 2, 3>(), make_march_variant<float, 256, 4, 256, 2, 3, 32>(),
+          make_march_variant<float, 256, 4, 256, 2, 3, 16>()};
 }
 
 constexpr int K2_NT = 256;

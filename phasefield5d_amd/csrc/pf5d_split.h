@@ -68,12 +68,13 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
     const long c = ((long)gx * n1 + gy) * n2 + gz;
     V4<T> xc{};
     if (valid) xc = ld4g<T>(sd.X + 4 * c);
-    // node box around the tile's first cell (the anchor): lower nodes one below its
+    // node box around the tile's centre cell (the anchor): lower nodes one below its
     // lower cube corners, so corners within +-1 of the anchor's are covered
     const int num = sd.tab.num;
     int lo[4];
     {
-      const V4<T> xa = ld4g<T>(sd.X + 4 * (((long)item.xa * n1 + item.y0) * n2 + item.z0));
+      const long ca = ((long)item.xa * n1 + item.y0 + item.ty / 2) * n2 + item.z0 + item.tz / 2;
+      const V4<T> xa = ld4g<T>(sd.X + 4 * ca);
       // (the box position only decides which points read LDS: clamped in
       // floating point first, so NaN lands on 0 and the conversion is in range)
       const float top = (float)(num - BOX4);
