@@ -39,7 +39,9 @@ int pf5d_device_info(int out[3]);
 
 /* ---- CALPHAD table (solver/interpolation.py:13-35 build_4d_grid output) ----
  * grid: host (num,num,num,num,nprops) float64 with NaN outside the simplex,
- * nprops == 9.  The library keeps a padded device copy in `prec`. */
+ * nprops == 9.  The library keeps a padded device copy in `prec`, and a cube
+ * table of the linear region (per hypercube its 16 corners contiguous, the
+ * order the interpolation reads them) for gathers that miss the LDS node box. */
 int pf5d_table_create(int prec, int num, int nprops, const double* grid_host, double resolution,
                       pf5d_table** out);
 int pf5d_table_destroy(pf5d_table* t);

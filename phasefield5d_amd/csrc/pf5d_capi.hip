@@ -132,6 +132,8 @@ struct pf5d_table {
   int prec, num;
   double resolution;
   void* data;
+  void* cubes;  // cube table of the linear region (null when num < 5)
+  int cube_s;
 };
 
 template <typename T>
@@ -142,7 +144,30 @@ static TableView<T> view(const pf5d_table* t) {
   v.inv_step = (T)(1.0 / t->resolution);
   v.thresh = (T)(1.0 - 4.0 * t->resolution - 1e-9);
   v.step = t->resolution;
+  v.cubes = reinterpret_cast<const T*>(t->cubes);
+  v.cube_s = t->cube_s;
   return v;
+}
+
+// one thread per dense cube position; cubes of the linear region copy their 16
+// corners into the cube table's order
+template <typename T>
+__global__ void k_build_cubes(const T* __restrict__ dense, int num, int S, T* __restrict__ cubes) {
+  constexpr int NP = TablePad<T>::value, VN = Vec16<T>::n;
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long m = num - 1;
+  if (id >= m * m * m * m) return;
+  const int i3 = (int)(id % m), i2 = (int)(id / m % m), i1 = (int)(id / (m * m) % m), i0 = (int)(id / (m * m * m));
+  if (i0 + i1 + i2 + i3 > S) return;
+  T* out = cubes + simplex_rank4(i0, i1, i2, i3, S) * CUBE_VALUES;
+  for (int b = 0; b < 16; ++b) {
+    const long row = (((long)(i0 + (b >> 3)) * num + i1 + ((b >> 2) & 1)) * num + i2 + ((b >> 1) & 1)) * num +
+                     i3 + (b & 1);
+    const T* node = dense + row * NP;
+    for (int g = 0; g < 8 / VN; ++g)
+      for (int e = 0; e < VN; ++e) out[(g * 16 + b) * VN + e] = node[g * VN + e];
+    out[128 + b] = node[8];
+  }
 }
 
 template <typename T>
@@ -154,6 +179,19 @@ static void upload_table(pf5d_table* t, const double* grid, int nprops) {
     for (int p = 0; p < nprops; ++p) h[i * NP + p] = (T)grid[i * nprops + p];
   t->data = dmalloc(h.size() * sizeof(T));
   HIPCHK(hipMemcpy(t->data, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  // the cube table: lower corners with index sum <= num - 5 hold every point
+  // of the linear region (sum X <= 1 - 4h - 1e-9)
+  t->cube_s = t->num - 5;
+  t->cubes = nullptr;
+  const long ncubes = simplex_count4(t->cube_s);
+  if (ncubes > 0) {
+    t->cubes = dmalloc((size_t)ncubes * CUBE_VALUES * sizeof(T));
+    const long m = t->num - 1, npos = m * m * m * m;
+    hipLaunchKernelGGL(k_build_cubes<T>, dim3((unsigned)((npos + 255) / 256)), dim3(256), 0, nullptr,
+                       reinterpret_cast<const T*>(t->data), t->num, t->cube_s, reinterpret_cast<T*>(t->cubes));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(nullptr));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1324,6 +1362,8 @@ int pf5d_table_create(int prec, int num, int nprops, const double* grid, double 
     t->num = num;
     t->resolution = resolution;
     t->data = nullptr;
+    t->cubes = nullptr;
+    t->cube_s = -1;
     if (prec == PF5D_F64)
       upload_table<double>(t.get(), grid, nprops);
     else
@@ -1335,6 +1375,7 @@ int pf5d_table_destroy(pf5d_table* t) {
   return guarded([&] {
     if (!t) return;
     dfree(t->data);
+    dfree(t->cubes);
     delete t;
   });
 }

@@ -36,7 +36,28 @@ struct TableView {
   T inv_step;      // 1/h (host: 1.0/h in FP64, rounded for FP32)
   T thresh;        // 1 - 4h - 1e-9
   double step;     // h, for node coordinates in the nearest search
+  const T* cubes;  // cube table (below), or null
+  int cube_s;      // cubes with lower-corner index sum <= cube_s are in it
 };
+
+// Cube table: for every hypercube of the linear region (lower corner i with
+// i0+i1+i2+i3 <= S, S = num - 5) the 16 corners' nine properties, stored in
+// the order the interpolation reads them, CUBE_VALUES values per cube: for
+// each 16-byte group g of properties, the 16 corners (c0000 .. c1111, i3
+// fastest) one after the other, then the ninth property of the 16 corners.  A
+// point's gather then reads 576 contiguous bytes (FP32) instead of 16 table
+// rows scattered over the 4D grid.  Cubes are ranked lexicographically among
+// the tuples with sum <= S.
+constexpr int CUBE_VALUES = 144;
+PF5D_HD long simplex_rank4(int i0, int i1, int i2, int i3, int S) {
+  // number of k-tuples of non-negative integers with sum <= r: C(r + k, k)
+  auto f2 = [](long r) { return (r + 1) * (r + 2) / 2; };
+  auto f3 = [](long r) { return (r + 1) * (r + 2) * (r + 3) / 6; };
+  auto f4 = [](long r) { return (r + 1) * (r + 2) * (r + 3) * (r + 4) / 24; };
+  const long R1 = S - i0, R2 = R1 - i1;
+  return (f4(S) - f4(R1)) + (f3(R1) - f3(R1 - i1)) + (f2(R2) - f2(R2 - i2)) + i3;
+}
+PF5D_HD long simplex_count4(int S) { return S < 0 ? 0 : (long)(S + 1) * (S + 2) * (S + 3) * (S + 4) / 24; }
 
 template <typename T>
 PF5D_DEV void load_corner(const T* p, T g[TablePad<T>::value]);
@@ -106,27 +127,33 @@ PF5D_DEV auto vget(const V& v, int i) {
 // the row of the lower corner; n1, n2, n3 are the element strides of axes 2, 1, 0
 // (the stride of axis 3 is one row).  The rows may live in the HBM table or in
 // an LDS copy of a box of nodes: the arithmetic is the same.
-template <typename T, bool FROM_LDS = false>
-PF5D_DEV void interp_core(const T* base, int n1, int n2, int n3, T t0, T t1, T t2, T t3, T P[9]) {
-  constexpr int NP = TablePad<T>::value;
-  using VT = typename Vec16<T>::type;
-  constexpr int VN = Vec16<T>::n;
-  constexpr int NB = (9 + VN - 1) / VN;  // batches: 3 (FP32) or 5 (FP64)
+// the 16 corner weights ((f0 f1) f2) f3 as a product tree: same roundings as the
+// reference's left-to-right product (interpolation.py:126-134), 28 multiplies
+template <typename T>
+PF5D_DEV void corner_weights(T t0, T t1, T t2, T t3, T w[16]) {
   const T s0 = T(1) - t0, s1 = T(1) - t1, s2 = T(1) - t2, s3 = T(1) - t3;
-  // eight row-pair pointers (b0 b1 b2); the b3 neighbour and the property column
-  // are compile-time offsets from them
-  const T* rp[8];
-#pragma unroll
-  for (int r = 0; r < 8; ++r) rp[r] = base + ((r >> 2) & 1) * n3 + ((r >> 1) & 1) * n2 + (r & 1) * n1;
-  // weights ((f0 f1) f2) f3 as a product tree: same roundings as the reference's
-  // left-to-right product (interpolation.py:126-134), 28 multiplies
-  T w01[4], w012[8], w[16];
+  T w01[4], w012[8];
 #pragma unroll
   for (int a = 0; a < 4; ++a) w01[a] = ((a >> 1) ? t0 : s0) * ((a & 1) ? t1 : s1);
 #pragma unroll
   for (int a = 0; a < 8; ++a) w012[a] = w01[a >> 1] * ((a & 1) ? t2 : s2);
 #pragma unroll
   for (int b = 0; b < 16; ++b) w[b] = w012[b >> 1] * ((b & 1) ? t3 : s3);
+}
+
+template <typename T, bool FROM_LDS = false>
+PF5D_DEV void interp_core(const T* base, int n1, int n2, int n3, T t0, T t1, T t2, T t3, T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  using VT = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  constexpr int NB = (9 + VN - 1) / VN;  // batches: 3 (FP32) or 5 (FP64)
+  // eight row-pair pointers (b0 b1 b2); the b3 neighbour and the property column
+  // are compile-time offsets from them
+  const T* rp[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) rp[r] = base + ((r >> 2) & 1) * n3 + ((r >> 1) & 1) * n2 + (r & 1) * n1;
+  T w[16];
+  corner_weights<T>(t0, t1, t2, t3, w);
   // full 16-byte batches, then the ninth property alone (no padding fetched)
 #pragma unroll
   for (int g = 0; g < 8 / VN; ++g) {
@@ -168,6 +195,51 @@ PF5D_DEV void interp_core(const T* base, int n1, int n2, int n3, T t0, T t1, T t
   (void)NB;
 }
 
+// interp_core's arithmetic on one cube of the cube table (contiguous corners)
+template <typename T>
+PF5D_DEV void interp_cube(const T* cube, T t0, T t1, T t2, T t3, T P[9]) {
+  using VT = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  T w[16];
+  corner_weights<T>(t0, t1, t2, t3, w);
+#pragma unroll
+  for (int g = 0; g < 8 / VN; ++g) {
+    VT c[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) c[b] = gld<VT>(cube + (g * 16 + b) * VN);
+#pragma unroll
+    for (int e = 0; e < VN; ++e) {
+      T acc = w[0] * vget(c[0], e);
+#pragma unroll
+      for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], vget(c[b], e), acc);
+      P[g * VN + e] = acc;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  T c[16];
+#pragma unroll
+  for (int b = 0; b < 16; ++b) c[b] = gld<T>(cube + 128 + b);
+  T acc = w[0] * c[0];
+#pragma unroll
+  for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], c[b], acc);
+  P[8] = acc;
+}
+
+// interpolation from the table in HBM for lower corner (i0..i3): the cube
+// table when the cube is in it, the dense rows otherwise
+template <typename T>
+PF5D_DEV void interp_gather(const TableView<T>& tv, int i0, int i1, int i2, int i3, T t0, T t1, T t2, T t3,
+                            T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  if (tv.cubes && i0 + i1 + i2 + i3 <= tv.cube_s) {
+    interp_cube<T>(tv.cubes + simplex_rank4(i0, i1, i2, i3, tv.cube_s) * CUBE_VALUES, t0, t1, t2, t3, P);
+  } else {
+    const int num = tv.num;
+    const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
+    interp_core<T, false>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);
+  }
+}
+
 template <typename T>
 PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
   constexpr int NP = TablePad<T>::value;
@@ -178,8 +250,8 @@ PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
   axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
   axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
   axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
-  const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
-  interp_core<T>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);
+  interp_gather<T>(tv, i0, i1, i2, i3, t0, t1, t2, t3, P);
+  (void)NP;
 }
 
 // A box of table nodes [lo_d, lo_d + ext_d) staged in LDS, rows of NPAD elements
@@ -227,8 +299,7 @@ PF5D_DEV void interp9_staged(const TableView<T>& tv, const T* box, const NodeBox
                            (i3 - nb.lo[3])) * NP;
     interp_core<T, true>(base, nb.st[2] * NP, nb.st[1] * NP, nb.st[0] * NP, t0, t1, t2, t3, P);
   } else {
-    const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
-    interp_core<T, false>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);
+    interp_gather<T>(tv, i0, i1, i2, i3, t0, t1, t2, t3, P);
   }
 }
 
@@ -281,8 +352,7 @@ PF5D_DEV void interp9_box4(const TableView<T>& tv, const T* box, const int lo[4]
     const T* base = box + (d0 * L::S0 + d1 * L::S1 + d2 * L::S2 + d3 * L::S3) * VN;
     interp_core<T, true>(base, L::S2 * VN, L::S1 * VN, L::S0 * VN, t0, t1, t2, t3, P);
   } else {
-    const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
-    interp_core<T, false>(base, num * NP, num * num * NP, num * num * num * NP, t0, t1, t2, t3, P);
+    interp_gather<T>(tv, i0, i1, i2, i3, t0, t1, t2, t3, P);
   }
 }
 
@@ -362,36 +432,31 @@ PF5D_DEV NodeBox stage_node_box(const TableView<T>& tab, int v[8], T* box, int* 
   return nb;
 }
 
+// Nearest valid table node (the reference's KD-tree query on the table rows,
+// interpolation.py:203-209): the squared distance is separable, so the nearest
+// grid node is the per-axis nearest, one of {floor(u), floor(u) + 1} on every
+// axis.  Those 16 nodes are scanned first (in the order and with the tie rule of
+// the full scan below); when the per-axis nearest node is a valid one (inside
+// the simplex) it is the answer.  Only otherwise -- the point lies beyond the
+// simplex face, whose nearest nodes can be further out -- are the 4^4 nodes
+// around it scanned.
 template <typename T>
-PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
-  constexpr int NP = TablePad<T>::value;
-  const int num = tv.num;
-  const double h = tv.step;
-  int lo[4];
-#pragma unroll
-  for (int d = 0; d < 4; ++d) {
-    double u = (double)X[d] / h;
-    int f = (u <= 0.0) ? 0 : (int)u;
-    if (f > num - 1) f = num - 1;
-    lo[d] = f - 1;
-  }
-  double best = 1e300;
-  long bidx = -1;
-  // rare path (outside the linear region): kept rolled so that it does not
-  // raise the register budget of the kernels it is inlined into
+PF5D_DEV void nearest_scan(const T X[4], int num, double h, const int lo[4], int a0, int a1, double& best,
+                           long& bidx) {
 #pragma unroll 1
-  for (int a = 0; a < 4; ++a) {
+  for (int a = a0; a < a1; ++a) {
     int n0 = lo[0] + a;
     if (n0 < 0 || n0 >= num) continue;
 #pragma unroll 1
-    for (int b = 0; b < 4; ++b) {
+    for (int b = a0; b < a1; ++b) {
       int n1 = lo[1] + b;
       if (n1 < 0 || n1 >= num || n0 + n1 > num - 1) continue;
 #pragma unroll 1
-      for (int c = 0; c < 4; ++c) {
+      for (int c = a0; c < a1; ++c) {
         int n2 = lo[2] + c;
         if (n2 < 0 || n2 >= num || n0 + n1 + n2 > num - 1) continue;
-        for (int e = 0; e < 4; ++e) {
+#pragma unroll 1
+        for (int e = a0; e < a1; ++e) {
           int n3 = lo[3] + e;
           if (n3 < 0 || n3 >= num || n0 + n1 + n2 + n3 > num - 1) continue;
           double d0 = (double)X[0] - n0 * h, d1 = (double)X[1] - n1 * h;
@@ -404,6 +469,36 @@ PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
         }
       }
     }
+  }
+}
+
+template <typename T>
+PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
+  constexpr int NP = TablePad<T>::value;
+  const int num = tv.num;
+  const double h = tv.step;
+  int lo[4], near_sum = 0;
+  bool near_in = true;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    double u = (double)X[d] / h;
+    int f = (u <= 0.0) ? 0 : (int)u;
+    if (f > num - 1) f = num - 1;
+    lo[d] = f - 1;
+    // per-axis nearest of the candidates f, f + 1 (ties to f, as the scan keeps the first)
+    const double df = (double)X[d] - f * h, dg = (double)X[d] - (f + 1) * h;
+    const int r = (f + 1 < num && dg * dg < df * df) ? f + 1 : f;
+    near_sum += r;
+    near_in = near_in && r >= 0;
+  }
+  near_in = near_in && near_sum <= num - 1;
+  double best = 1e300;
+  long bidx = -1;
+  nearest_scan<T>(X, num, h, lo, 1, 3, best, bidx);
+  if (!near_in) {
+    best = 1e300;
+    bidx = -1;
+    nearest_scan<T>(X, num, h, lo, 0, 4, best, bidx);
   }
   if (bidx < 0) {
 #pragma unroll
