@@ -12,6 +12,8 @@ Reported:
   fp64_plain   the per-function drop-in FP64 path (one device call per reference
                function, host dt from abs_max each step)
   error        max / RMS |X_fp32 - X_fp64| after the timed steps (same init)
+  fp32_developed_field  the FP32 step on a decomposed-looking field (smooth
+               modes of amplitude 0.05 per component), as later in a run
   roofline     the dominant kernel of the FP32 step: with the two-pass stencil
                (the autotuner's usual pick) the potentials pass, algorithmic
                bytes 56 B/cell (X 16 + psi 4 read; mu 16 + Mob 20 written); with
@@ -20,6 +22,7 @@ Reported:
   cpu_baseline C/OpenMP FP64 restatement of the reference loop + scipy.fft
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -181,6 +184,26 @@ def main():
     if not (st32["status"] & 1) or st32["steps"] != args.warmup + args.steps:
         raise RuntimeError(f"fp32 simulation stopped early: {st32}")
 
+    # the same step on a decomposed-looking field (neighbouring cells in different
+    # table hypercubes, some past the linear region), as later in a run
+    from phasefield5d_amd.workloads import developed_field
+    ed = Engine("fp32")
+    pbd = dataclasses.replace(pb, X_init=developed_field(pb, amp=0.05, seed=4))
+    sd_ = ed.add_problem(pbd)
+    ed.set_tiling(info["variant"], info["cx"])
+    ed.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
+    ed.step(5)
+    ed.sync()
+    t0 = time.perf_counter()
+    ed.step(50)
+    ed.sync()
+    eld = time.perf_counter() - t0
+    developed = {"amplitude": 0.05, "steps": 50, "ms_per_step": 1e3 * eld / 50, "cell_updates_per_s": ncell * 50 / eld,
+                 "k1_potentials_ms": ed.time_kernel("k1_potentials", reps=10) if split else None,
+                 "active": bool(ed.state(sd_)["status"] & 1)}
+    ed.close()
+    log("fp32 developed field", developed)
+
     mass0 = pb.X_init.reshape(-1, 4).sum(axis=0)
     drift32 = np.abs(st32["mass"] - mass0) / mass0
     v64 = k1_64 = el64 = vplain = None
@@ -245,6 +268,7 @@ def main():
         "error_vs_fp64": {"max": float(diff.max()), "rms": float(np.sqrt((diff ** 2).mean())),
                           "steps": args.warmup + args.steps},
         "mass_drift_fp32": drift32.tolist(),
+        "fp32_developed_field": developed,
         "stage_ms": {"k1": k1_ms, "k1_potentials": kp_ms, "k1_flux": kf_ms, "fft": fft_ms, "k2": k2_ms,
                      "dt": dt_ms},
         "launch": info,

@@ -692,7 +692,9 @@ static K1Variant<T> make_split_variant() {
 }
 // two-pass form with the marching flux pass (FNT threads, CPL cells per lane,
 // FMINW waves per SIMD at least)
-template <typename T, int NT, int MINW, int FNT, int CPL, int FMINW = 1, int BZ = 64>
+// (the potentials pass's tiles are 16 x 16 cells: a compact tile keeps a
+// developed field's cells close to its node box's anchor)
+template <typename T, int NT, int MINW, int FNT, int CPL, int FMINW = 1, int BZ = 16>
 static K1Variant<T> make_march_variant() {
   return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL, FMINW>};
 }
@@ -706,7 +708,7 @@ std::vector<K1Variant<double>> k1_variants<double>() {
           make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, 0>(),
           make_split_variant<double, 256, 2, 1>(), make_split_variant<double, 256, 2, 1, 32>(),
           make_split_variant<double, 256, 2, 2>(), make_march_variant<double, 256, 2, 256, 1>(),
-          make_march_variant<double, 256, 2, 256, 1, 3>(), make_march_variant<double, 256, 2, 256, 1, 3, 16>()};
+          make_march_variant<double, 256, 2, 256, 1, 3>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
@@ -717,8 +719,7 @@ std::vector<K1Variant<float>> k1_variants<float>() {
           make_split_variant<float, 256, 4, 1>(), make_split_variant<float, 256, 4, 1, 32>(),
           make_split_variant<float, 256, 4, 2>(), make_march_variant<float, 256, 4, 256, 1>(),
           make_march_variant<float, 256, 4, 256, 2>(), make_march_variant<float, 256, 4, 512, 2>(),
-          make_march_variant<float, 256, 4, 256, 2, 3>(), make_march_variant<float, 256, 4, 256, 2, 3, 32>(),
-          make_march_variant<float, 256, 4, 256, 2, 3, 16>()};
+          make_march_variant<float, 256, 4, 256, 2, 3>(), make_march_variant<float, 256, 4, 256, 2, 3, 64>()};
 }
 
 constexpr int K2_NT = 256;

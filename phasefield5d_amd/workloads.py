@@ -106,3 +106,27 @@ def small_config(dim, n, X0="0.1,0.2,0.2,0.3", kappa=5e-16, seed=0, elastic=True
     if not elastic:
         args = [a for a in args if a != "--include_cubic_anisotropy"] + ["--no_include_cubic_anisotropy"]
     return parse_args_to_config(args)
+
+
+def developed_field(pb, amp=0.05, seed=1):
+    """A composition field that looks decomposed: X0 plus, per component, three
+    smooth periodic modes (wavenumbers 1-3 per axis, random phases) of total
+    amplitude ``amp``.  Used to time the step on fields like those later in a run,
+    where neighbouring cells no longer share a table hypercube."""
+    rng = np.random.default_rng(seed)
+    shape = tuple(pb.shape)
+    axes = [np.arange(n) for n in shape]
+    X = np.broadcast_to(pb.X0, shape + (4,)).copy()
+    for s in range(4):
+        f = np.zeros(shape)
+        for _ in range(3):
+            k = [int(rng.integers(1, 4)) for _ in shape]
+            ph = rng.uniform(0, 2 * np.pi)
+            arg = ph
+            for d in range(len(shape)):
+                sh = [1] * len(shape)
+                sh[d] = shape[d]
+                arg = arg + (2 * np.pi * k[d] / shape[d]) * axes[d].reshape(sh)
+            f += np.cos(arg)
+        X[..., s] += amp / 3 * f
+    return X
