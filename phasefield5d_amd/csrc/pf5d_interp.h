@@ -85,18 +85,16 @@ PF5D_DEV void load_corner<float>(const float* p, float g[12]) {
   }
 }
 
+// interpolation.py:110-121: u <= 0 -> (0, 0); u >= num-1 -> (num-2, 1); else
+// (int(u), u - int(u)).  Branch-free: the floor clamped to [0, num-2], then
+// t = u - i clamped to [0, 1] (+0 turns a -0 into +0 as the first case has it).
+// In the open interval both give floor(u) and the same subtraction; below and
+// above, the clamps give exactly 0 and 1.
 template <typename T>
 PF5D_DEV void axis_index(T u, int num, int& i, T& t) {
-  if (u <= T(0)) {
-    i = 0;
-    t = T(0);
-  } else if (u >= T(num - 1)) {
-    i = num - 2;
-    t = T(1);
-  } else {
-    i = (int)u;
-    t = u - (T)i;
-  }
+  const T fi = fmin(fmax(floor(u), T(0)), T(num - 2));
+  i = (int)fi;
+  t = fmin(fmax(u - fi, T(0)), T(1)) + T(0);
 }
 
 // 16-byte vector of the table element type
