@@ -136,6 +136,14 @@ int pf5d_engine_destroy(pf5d_engine* e);
 /* X_host: (ncell,4) float64; bhat_host as for pf5d_elastic_create; returns slot */
 int pf5d_engine_add(pf5d_engine* e, const pf5d_sim_desc* d, const double* X_host, const double* bhat_host,
                     const pf5d_table* tab, int* slot);
+/* pf5d_engine_add with the initial field drawn on the device instead of by
+ * apply_fluctuations on the host (solver/system.py:56-61, same distribution):
+ * X = X0 + fluctuation * (noise - mean over the five components)[1:], noise
+ * uniform in [-1, 1) from a counter-based generator (SplitMix64 of seed and the
+ * value's index), so large fields need no host draw and no upload.  The same
+ * seed gives the same FP64 field in either precision. */
+int pf5d_engine_add_generated(pf5d_engine* e, const pf5d_sim_desc* d, unsigned long long seed, double fluctuation,
+                              const double* bhat_host, const pf5d_table* tab, int* slot);
 int pf5d_engine_remove(pf5d_engine* e, int slot);
 /* K1 tile: variant index (pf5d_engine_num_variants), x chunk (3D), per dimension.
  * info (5*n ints, may be NULL), per variant: by, bz, nt (by == 0: two-pass form),

@@ -50,6 +50,7 @@ _SIGS = {
     "pf5d_engine_create": (i32, [i32, vp, C.POINTER(vp)]),
     "pf5d_engine_destroy": (i32, [vp]),
     "pf5d_engine_add": (i32, [vp, C.POINTER(SimDesc), P_f64, P_f64, vp, P_i32]),
+    "pf5d_engine_add_generated": (i32, [vp, C.POINTER(SimDesc), C.c_uint64, f64, P_f64, vp, P_i32]),
     "pf5d_engine_remove": (i32, [vp, i32]),
     "pf5d_engine_num_variants": (i32, [vp, P_i32, P_i32]),
     "pf5d_engine_set_tiling": (i32, [vp, i32, i32]),

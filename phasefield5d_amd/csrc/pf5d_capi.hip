@@ -580,6 +580,68 @@ static T* upload_bhat(int dim, const long* shape, const double* bhat_host) {
 }
 
 // ---------------------------------------------------------------------------
+// initial field generated on the device
+// ---------------------------------------------------------------------------
+// SplitMix64 (Steele, Lea and Flood, OOPSLA 2014) of a counter: uniform double
+// in [0, 1) with 53 random bits, value number i of stream seed
+PF5D_HD double unit_uniform(unsigned long long seed, unsigned long long i) {
+  unsigned long long z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+constexpr int INIT_NT = 256;
+// X = X0 + f (noise - mean(noise))[1:], noise = 2u - 1 over the five components
+// (the distribution of apply_fluctuations, system.py:56-61, from a counter-based
+// generator); FP32 mode stores the nearest FP32 value and the remainder.  Each
+// block leaves Neumaier partial sums of the FP64 field (sum, compensation per
+// component) in part[8 * block ...] for the FP64 mean.
+template <typename T>
+__global__ void __launch_bounds__(INIT_NT) k_init_field(T* __restrict__ X, T* __restrict__ Xlo, long ncell,
+                                                        double x00, double x01, double x02, double x03, double f,
+                                                        unsigned long long seed, double* __restrict__ part) {
+  const double X0[4] = {x00, x01, x02, x03};
+  double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
+  for (long c = (long)blockIdx.x * INIT_NT + threadIdx.x; c < ncell; c += (long)gridDim.x * INIT_NT) {
+    double n[5], m = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      n[k] = (unit_uniform(seed, (unsigned long long)c * 5 + k) - 0.5) * 2.0;
+      m += n[k];
+    }
+    m /= 5.0;
+    V4<T> hi, lo;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double x = X0[a] + f * (n[a + 1] - m);
+      hi.v[a] = (T)x;
+      lo.v[a] = (T)(x - (double)hi.v[a]);
+      const double t = sum[a] + x;
+      cmp[a] += (fabs(sum[a]) >= fabs(x)) ? ((sum[a] - t) + x) : ((x - t) + sum[a]);
+      sum[a] = t;
+    }
+    st4g<T>(X + 4 * c, hi);
+    if (Xlo) st4g<T>(Xlo + 4 * c, lo);
+  }
+  __shared__ double red[INIT_NT / 64][8];
+  double v[8] = {sum[0], cmp[0], sum[1], cmp[1], sum[2], cmp[2], sum[3], cmp[3]};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] += __shfl_xor(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 8; ++k) red[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double r = 0.0;
+    for (int w = 0; w < INIT_NT / 64; ++w) r += red[w][threadIdx.x];
+    part[8 * blockIdx.x + threadIdx.x] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // plain elastic updater
 // ---------------------------------------------------------------------------
 struct pf5d_elastic {
@@ -638,6 +700,8 @@ static void elastic_apply(pf5d_elastic* e, void* mu, long mstride, const void* X
 struct EngineBase {
   virtual ~EngineBase() {}
   virtual void add(const pf5d_sim_desc* d, const double* X, const double* bhat, const pf5d_table* tab, int* slot) = 0;
+  virtual void add_generated(const pf5d_sim_desc* d, unsigned long long seed, double fluctuation, const double* bhat,
+                             const pf5d_table* tab, int* slot) = 0;
   virtual void remove(int slot) = 0;
   virtual void step(int n) = 0;
   virtual void get_state(int slot, double* out) = 0;
@@ -841,6 +905,39 @@ struct Engine : EngineBase {
     cap = nc;
   }
   void add(const pf5d_sim_desc* d, const double* Xh, const double* bhat, const pf5d_table* tab, int* slot) override {
+    add_with(d, bhat, tab, slot, [&](Sim& s) { upload_X(s, Xh); });
+  }
+  void add_generated(const pf5d_sim_desc* d, unsigned long long seed, double fluctuation, const double* bhat,
+                     const pf5d_table* tab, int* slot) override {
+    add_with(d, bhat, tab, slot, [&](Sim& s) {
+      const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((s.ncell + INIT_NT - 1) / INIT_NT, 4096));
+      double* part = (double*)dmalloc((size_t)grid * 8 * sizeof(double));
+      try {
+        hipLaunchKernelGGL(k_init_field<T>, dim3(grid), dim3(INIT_NT), 0, stream, s.X, s.Xlo, s.ncell, d->X0[0],
+                           d->X0[1], d->X0[2], d->X0[3], fluctuation, seed, part);
+        HIPCHK(hipGetLastError());
+        std::vector<double> h((size_t)grid * 8);
+        HIPCHK(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int a = 0; a < 4; ++a) {
+          double sum = 0, c = 0;
+          for (unsigned b = 0; b < grid; ++b)
+            for (int q = 0; q < 2; ++q) {
+              const double v = h[8 * b + 2 * a + q], t = sum + v;
+              c += (std::fabs(sum) >= std::fabs(v)) ? ((sum - t) + v) : ((v - t) + sum);
+              sum = t;
+            }
+          s.mean0[a] = (sum + c) / (double)s.ncell;
+        }
+      } catch (...) {
+        dfree(part);
+        throw;
+      }
+      dfree(part);
+    });
+  }
+  template <typename Fill>
+  void add_with(const pf5d_sim_desc* d, const double* bhat, const pf5d_table* tab, int* slot, Fill fill) {
     require(d->dim >= 1 && d->dim <= 3, "dim must be 1, 2 or 3", PF5D_ENOTIMPL);
     require(tab && tab->prec == (sizeof(T) == 8 ? PF5D_F64 : PF5D_F32), "table precision does not match engine");
     require(tab->num >= BOX4, "the engine needs a table with at least 4 nodes per axis");
@@ -886,7 +983,7 @@ struct Engine : EngineBase {
         s.bhat = upload_bhat<T>(d->dim, d->shape, bhat);
         s.bytes += ncell * 2 * sizeof(T) + spectrum_size<T>(d->dim, d->shape) * (sizeof(cx<T>) + sizeof(T));
       }
-      upload_X(s, Xh);
+      fill(s);
       HIPCHK(hipMemset(s.dX, 0, ncell * 4 * sizeof(T)));
     } catch (...) {
       free_sim(s);
@@ -1698,6 +1795,13 @@ int pf5d_engine_add(pf5d_engine* e, const pf5d_sim_desc* d, const double* X, con
   return guarded([&] {
     require(e && d && X && slot, "engine_add: null argument");
     e->impl->add(d, X, bhat, tab, slot);
+  });
+}
+int pf5d_engine_add_generated(pf5d_engine* e, const pf5d_sim_desc* d, unsigned long long seed, double fluctuation,
+                              const double* bhat, const pf5d_table* tab, int* slot) {
+  return guarded([&] {
+    require(e && d && slot, "engine_add_generated: null argument");
+    e->impl->add_generated(d, seed, fluctuation, bhat, tab, slot);
   });
 }
 int pf5d_engine_remove(pf5d_engine* e, int slot) {

@@ -41,7 +41,9 @@ class Engine:
 
     # -- membership ------------------------------------------------------------
     def add_problem(self, pb, X=None, t_end=math.inf, oob_interval=100, dt0=None):
-        """Add a set-up problem (see problem.build_problem); returns its slot."""
+        """Add a set-up problem (see problem.build_problem); returns its slot.  A problem
+        without a host field (X_init None) and with ``extra["device_init"] = (seed,
+        fluctuation)`` has its initial field drawn on the device."""
         cfg = pb.cfg
         d = _lib.SimDesc()
         d.dim = len(pb.shape)
@@ -59,15 +61,23 @@ class Engine:
         d.t_end = t_end if math.isfinite(t_end) else 1e300
         d.oob_interval = oob_interval
         d.tol_soft, d.tol_hard = 1e-5, 1e-3
-        Xh = np.ascontiguousarray(pb.X_init if X is None else X, dtype=np.float64)
-        if Xh.shape != tuple(pb.shape) + (4,):
-            raise ValueError(f"X shape {Xh.shape} != {tuple(pb.shape) + (4,)}")
         bh = np.ascontiguousarray(pb.Bhat, dtype=np.float64) if pb.Bhat is not None else None
         tab = device_table(pb.data_grid, cfg.resolution, self.prec)
         slot = C.c_int()
-        _lib.check(_lib.lib().pf5d_engine_add(self.h, C.byref(d), _lib.ptr_f64(Xh),
-                                             _lib.ptr_f64(bh) if bh is not None else None, tab.handle,
-                                             C.byref(slot)))
+        gen = pb.extra.get("device_init") if X is None and pb.X_init is None else None
+        if gen is not None:
+            # the field is drawn on the device: (seed, fluctuation)
+            seed, fluct = gen
+            _lib.check(_lib.lib().pf5d_engine_add_generated(self.h, C.byref(d), int(seed), float(fluct),
+                                                           _lib.ptr_f64(bh) if bh is not None else None,
+                                                           tab.handle, C.byref(slot)))
+        else:
+            Xh = np.ascontiguousarray(pb.X_init if X is None else X, dtype=np.float64)
+            if Xh.shape != tuple(pb.shape) + (4,):
+                raise ValueError(f"X shape {Xh.shape} != {tuple(pb.shape) + (4,)}")
+            _lib.check(_lib.lib().pf5d_engine_add(self.h, C.byref(d), _lib.ptr_f64(Xh),
+                                                 _lib.ptr_f64(bh) if bh is not None else None, tab.handle,
+                                                 C.byref(slot)))
         self.slots[slot.value] = (pb, tab)
         return slot.value
 

@@ -149,7 +149,7 @@ def build_problem(cfg, table=None, verbose=False, X_init=None):
         if not rank1:
             raise NotImplementedError("non-rank-1 elastic coupling")
     if X_init is None:
-        X_init = apply_fluctuations(X0, shape, cfg.fluctuation, seed=cfg.seed).astype(np.float64)
+        X_init = np.asarray(apply_fluctuations(X0, shape, cfg.fluctuation, seed=cfg.seed), dtype=np.float64)
     kappa = np.asarray(cfg.kappa_i, dtype=float).reshape(1, 4) * cfg.kappa_value
     return Problem(cfg=cfg, rows=rows, data_grid=grid, tree=tree, calphad_values=values,
                    hessian_max=hessian_max, mobility_max=mobility_max, cij=(c11, c12, c44), molar_volume=vm,

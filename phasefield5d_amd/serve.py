@@ -70,8 +70,10 @@ def footprint_bytes(pb, precision):
 
 
 class Server:
-    def __init__(self, precision="fp32", hbm_budget=None, poll_steps=20, setup_workers=4, autotune=False):
+    def __init__(self, precision="fp32", hbm_budget=None, poll_steps=20, setup_workers=4, autotune=False,
+                 fast_init=True):
         self.engine = Engine(precision)
+        self.fast_init = fast_init  # initial fields drawn on the device (Engine.add_problem)
         self.precision = precision
         if hbm_budget is None:
             import torch
@@ -85,7 +87,13 @@ class Server:
         self.log = []
 
     def _setup(self, req):
-        pb = build_problem(req.cfg, table=req.table)
+        if self.fast_init:
+            # the initial field is drawn on the device when the request is admitted
+            pb = build_problem(req.cfg, table=req.table, X_init=np.empty((0,)))
+            pb.X_init = None
+            pb.extra["device_init"] = (req.cfg.seed, req.cfg.fluctuation)
+        else:
+            pb = build_problem(req.cfg, table=req.table)
         t_end = req.t_end_ctld * pb.ctld
         if req.max_steps is not None:
             t_end = min(t_end, req.max_steps * pb.dt_cfl)

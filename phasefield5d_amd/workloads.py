@@ -130,3 +130,4 @@ def developed_field(pb, amp=0.05, seed=1):
             f += np.cos(arg)
         X[..., s] += amp / 3 * f
     return X
+

@@ -62,3 +62,48 @@ def test_admission_respects_budget():
     spans = sorted((r.admitted, r.finished) for r in done)
     for (a0, f0), (a1, f1) in zip(spans, spans[1:]):
         assert a1 >= f0 - 1e-6  # never two resident at once
+
+
+def _splitmix_uniform(seed, i):
+    """The published SplitMix64 step of counter i (Steele, Lea, Flood 2014), 53 bits."""
+    m = (1 << 64) - 1
+    z = (seed + (i + 1) * 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    return (z >> 11) * (1.0 / 9007199254740992.0)
+
+
+def test_device_drawn_initial_field():
+    """A field drawn on the device: bit-equal to a restatement of the generator in
+    FP64; the FP32 engine holds the same field as value + remainder; the engine's
+    FP64 mass equals the field's; a served request with a device-drawn field equals
+    the same request run alone."""
+    import dataclasses
+    from phasefield5d_amd.engine import Engine
+    base = problem(2, 16, seed=3)
+    pb = dataclasses.replace(base, X_init=None, extra={"device_init": (12345, 1e-3)})
+    e64, e32 = Engine("fp64"), Engine("fp32")
+    s64, s32 = e64.add_problem(pb), e32.add_problem(pb)
+    X64, X32 = e64.field(s64, "X"), e32.field(s32, "X")
+    mass64 = e64.state(s64)["mass"]
+    e64.close()
+    e32.close()
+    ref = np.empty((16 * 16, 4))
+    for c in range(16 * 16):
+        n = [(_splitmix_uniform(12345, 5 * c + k) - 0.5) * 2.0 for k in range(5)]
+        m = ((((n[0] + n[1]) + n[2]) + n[3]) + n[4]) / 5.0
+        ref[c] = [pb.X0[a] + 1e-3 * (n[a + 1] - m) for a in range(4)]
+    assert np.array_equal(X64.reshape(-1, 4), ref)
+    assert np.abs(X32 - X64).max() <= 1e-15
+    assert np.allclose(mass64, X64.reshape(-1, 4).sum(axis=0), rtol=1e-14, atol=0)
+    # served with the field drawn at admission == alone
+    from phasefield5d_amd.serve import Request, Server
+    r = Request(0, pb.cfg, None, arrival=0.0, max_steps=20)
+    r.problem, r.t_end = pb, 20 * pb.dt_cfl
+    srv = Server("fp32", poll_steps=5, setup_workers=1)
+    srv._setup = lambda req: req
+    done = srv.run([r])
+    srv.close()
+    X, st = _alone(pb, r.t_end, "fp32")
+    assert np.array_equal(done[0].X, X)
