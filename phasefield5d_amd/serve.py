@@ -87,6 +87,12 @@ class Server:
         self.log = []
 
     def _setup(self, req):
+        t_setup = time.perf_counter()
+        self._setup_problem(req)
+        req.extra["setup_s"] = time.perf_counter() - t_setup
+        return req
+
+    def _setup_problem(self, req):
         if self.fast_init:
             # the initial field is drawn on the device when the request is admitted
             pb = build_problem(req.cfg, table=req.table, X_init=np.empty((0,)))
@@ -136,8 +142,10 @@ class Server:
                     break
                 ready.pop(0)
                 req.slot = self.engine.add_problem(req.problem, t_end=req.t_end)
-                req.admitted = time.perf_counter() - t0
+                ta = time.perf_counter()
+                req.admitted = ta - t0
                 req.extra["bytes"] = need
+                req.extra["admit_s"] = time.perf_counter() - ta
                 self.used_bytes += need
                 running[req.slot] = req
                 admitted = True
@@ -226,9 +234,18 @@ def poisson_requests(rate, duration, seed=0, max_steps=500, audit_frac=0.05, t_e
 
 def summarize(done, wall):
     lat = np.array([r.latency for r in done])
+
+    def pct(key, q):
+        v = [r.extra[key] for r in done if key in r.extra]
+        return float(np.percentile(v, q)) if v else None
     cells = sum(r.problem.ncell * r.steps for r in done)
     errs = [r.audit_err for r in done if r.audit_err is not None]
+    # time from arrival to admission (host set-up, waiting for a set-up worker or for memory)
+    wait = [r.admitted - r.arrival for r in done]
     return {"requests": len(done), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "setup_p50_s": pct("setup_s", 50), "setup_p99_s": pct("setup_s", 99),
+            "admit_p50_s": pct("admit_s", 50), "admit_p99_s": pct("admit_s", 99),
+            "to_admission_p50_s": float(np.percentile(wait, 50)), "to_admission_p99_s": float(np.percentile(wait, 99)),
             "mean_s": float(lat.mean()), "wall_s": wall, "cell_updates": int(cells),
             "cell_updates_per_s": cells / wall, "audited": len(errs),
             "audit_max_err": max(errs) if errs else None,
