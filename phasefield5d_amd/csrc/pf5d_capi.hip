@@ -242,7 +242,7 @@ struct FftPlans {
           tw.push_back({(T)re, (T)im});
         }
       pl.root_off[p] = (int)roots.size();
-      if (!(R == 2 || R == 3 || R == 4 || R == 5 || R == 8))
+      if (fft_generic_radix(R))
         for (int m = 0; m < R; ++m) {
           double re, im;
           root_of_unity(m, R, re, im);
@@ -250,6 +250,8 @@ struct FftPlans {
         }
       Ns *= R;
     }
+    pl.nroot = (int)roots.size() - pl.root_off[0];
+    require(pl.n + pl.nroot == fft_head_slots(n), "FFT plan roots do not match the LDS layout");
     int id = (int)plans.size();
     plans.push_back(pl);
     index[n] = id;
@@ -359,7 +361,8 @@ struct FftBatch {
   // complex lines per item: two line buffers within the LDS budget (real-line
   // stages carry two lines per complex line)
   int pick_g(int n, long nlines, int kind) const {
-    const size_t avail = budget > (size_t)n * sizeof(cx<T>) ? budget - (size_t)n * sizeof(cx<T>) : 0;
+    const size_t head = (size_t)fft_head_slots(n) * sizeof(cx<T>);
+    const size_t avail = budget > head ? budget - head : 0;
     long g = (long)(avail / (2 * (size_t)line_stride(n) * sizeof(cx<T>)));
     g = std::max<long>(1, std::min<long>(g, 128));
     if (paired(kind)) g *= 2;
@@ -368,7 +371,7 @@ struct FftBatch {
   void push(int st, FftItem it, int n) {
     const size_t gl = paired(it.kind) ? (it.G + 1) / 2 : it.G;
     stage[st].push_back(it);
-    lds[st] = std::max(lds[st], ((size_t)n + 2 * gl * line_stride(n)) * sizeof(cx<T>));
+    lds[st] = std::max(lds[st], ((size_t)fft_head_slots(n) + 2 * gl * line_stride(n)) * sizeof(cx<T>));
   }
   // lines [0, nlines) with line l at base + l*ls; split into groups of G (a
   // multiple of align lines when G >= align)
@@ -740,6 +743,7 @@ struct K1Variant {
   int fcpl = 0, fnt = 0;
   void (*fmarch)(const SimDev<T>*, SimState*, T*, const K1Item*, int, const double*, const int*, int,
                  unsigned*) = nullptr;
+  int fminw = 0;  // marching pass: min waves per SIMD it was compiled for
   bool split() const { return by == 0; }
 };
 
@@ -760,7 +764,7 @@ static K1Variant<T> make_split_variant() {
 // developed field's cells close to its node box's anchor)
 template <typename T, int NT, int MINW, int FNT, int CPL, int FMINW = 1, int BZ = 16>
 static K1Variant<T> make_march_variant() {
-  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL, FMINW>};
+  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL, FMINW>, FMINW};
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -837,10 +841,22 @@ struct Engine : EngineBase {
     for (auto& v : variants)
       if (!v.split())
         (void)hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds);
+    // default tiling (until set_tiling): the marching two-pass form with 16 x 16
+    // potentials tiles and the flux pass at three waves per SIMD, the
+    // autotuner's pick for 3D fields from 64^3 to 256^3 in both precisions
+    // (about twice as fast there as the single-pass tilings); x chunks of 16
+    // planes; FFT lines of about six points per thread (32 KiB, 256 threads)
+    // and two (FP32) or four (FP64) parts per plane of the 128 x 128 plane passes
     variant = -1;
+    for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
+      if (variants[i].fcpl > 0 && variants[i].fminw == 3 && variants[i].bz == 16) variant = i;
     for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
       if (!variants[i].split() && variants[i].lds <= (size_t)devinfo().lds) variant = i;
     require(variant >= 0, "no stencil tiling fits this device's LDS");
+    fft.budget = 32 * 1024;
+    fft.nt = 256;
+    fft.parts = sizeof(T) == 4 ? 2 : 4;
+    fft.pnt = 1024;
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);

@@ -63,6 +63,7 @@ struct LinePlanDev {
   int radix[FFT_MAXPASS];
   int tw_off[FFT_MAXPASS];    // twiddles of pass p: tw[tw_off + k*(R-1) + (r-1)], k < Ns, 1 <= r < R
   int root_off[FFT_MAXPASS];  // roots exp(-2 pi i m / R), m < R (generic radices only)
+  int nroot;                  // roots of all generic passes (contiguous from root_off[0])
 };
 
 enum FftKind { FK_R2C = 0, FK_FWD = 1, FK_CONV = 2, FK_INV = 3, FK_C2R = 4, FK_LINE = 5 };
@@ -212,11 +213,65 @@ PF5D_HD constexpr int fft_min_radix(int n) {
 PF5D_DEV constexpr int lpad(int e) { return e + (e >> 3); }
 PF5D_HD constexpr int line_stride(int n) { return n + (n >> 3) + 1; }
 
-// one Stockham pass over G lines (line stride LS): src -> dst.  tw holds this
-// pass's twiddles (in LDS), roots the generic radix's roots of unity.
+// radices with hand-written butterflies; every other factor runs the direct DFT
+PF5D_HD constexpr bool fft_generic_radix(int R) { return !(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16); }
+// LDS slots ahead of an item's line buffers: n for the twiddles (n - 1 used),
+// then the roots of unity of the generic passes
+PF5D_HD constexpr int fft_head_slots(int n) {
+  int s = n;
+  for (int p = 0; fft_radix(n, p) != 0; ++p)
+    if (fft_generic_radix(fft_radix(n, p))) s += fft_radix(n, p);
+  return s;
+}
+
+// direct radix-R DFT pass, one output point per thread:
+//     y_q = x_0 + sum_{p >= 1} (x_p w_p) root^(p q mod R)
+// The twiddled inputs x_p w_p are formed once, in place in src (the pass's
+// source is dead after it), then every output sums over them; a butterfly's
+// outputs sit on adjacent threads, so their LDS reads of x_p coincide.  Same
+// operations and order per output as forming x_p w_p inside the sum.
 template <typename T, int NT>
-PF5D_DEV void stockham_pass(const cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int n, int LS, int G, int R,
-                            int Ns, const cx<T>* __restrict__ tw, const cx<T>* __restrict__ roots) {
+PF5D_DEV void generic_pass(cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int n, int LS, int G, int R, int Ns,
+                           const cx<T>* __restrict__ tw, const cx<T>* __restrict__ roots) {
+  const int nb = n / R, tot = G * n;
+  const float rn = frcp(n), rnb = frcp(nb), rns = frcp(Ns), rr = frcp(R);
+  for (int u = threadIdx.x; u < tot; u += NT) {
+    int g, m, p, j, q, k;
+    fdivmod(u, n, rn, g, m);
+    fdivmod(m, nb, rnb, p, j);
+    if (p == 0) continue;
+    fdivmod(j, Ns, rns, q, k);
+    cx<T>& x = src[g * LS + lpad(m)];
+    x = cmul(x, tw[k * (R - 1) + p - 1]);
+  }
+  __syncthreads();
+  for (int u = threadIdx.x; u < tot; u += NT) {
+    int b, qq, g, j, q, k;
+    fdivmod(u, R, rr, b, qq);
+    fdivmod(b, nb, rnb, g, j);
+    fdivmod(j, Ns, rns, q, k);
+    const cx<T>* s = src + g * LS;
+    cx<T> acc = s[lpad(j)];
+    int e = 0;
+#pragma unroll 4
+    for (int p = 1; p < R; ++p) {
+      e += qq;
+      if (e >= R) e -= R;
+      acc = cadd(acc, cmul(s[lpad(j + p * nb)], roots[e]));
+    }
+    dst[g * LS + lpad(q * Ns * R + k + qq * Ns)] = acc;
+  }
+}
+
+// one Stockham pass over G lines (line stride LS): src -> dst.  tw holds this
+// pass's twiddles, roots the generic radix's roots of unity (both in LDS).
+template <typename T, int NT>
+PF5D_DEV void stockham_pass(cx<T>* __restrict__ src, cx<T>* __restrict__ dst, int n, int LS, int G, int R, int Ns,
+                            const cx<T>* __restrict__ tw, const cx<T>* __restrict__ roots) {
+  if (fft_generic_radix(R)) {
+    generic_pass<T, NT>(src, dst, n, LS, G, R, Ns, tw, roots);
+    return;
+  }
   const int nb = n / R;
   const int tot = G * nb;
   const float rnb = frcp(nb), rns = frcp(Ns);
@@ -236,7 +291,7 @@ PF5D_DEV void stockham_pass(const cx<T>* __restrict__ src, cx<T>* __restrict__ d
       dft16(v);
 #pragma unroll
       for (int r = 0; r < 16; ++r) d[lpad(ob + r * Ns)] = v[r];
-    } else if (R == 2 || R == 3 || R == 4 || R == 5 || R == 8) {
+    } else {
       cx<T> v[8];
       v[0] = s[lpad(j)];
 #pragma unroll
@@ -255,18 +310,6 @@ PF5D_DEV void stockham_pass(const cx<T>* __restrict__ src, cx<T>* __restrict__ d
 #pragma unroll
       for (int r = 0; r < 8; ++r)
         if (r < R) d[lpad(ob + r * Ns)] = v[r];
-    } else {
-      // direct radix-R DFT: y_q = sum_p (x_p w_p) root^(p q mod R)
-      for (int qq = 0; qq < R; ++qq) {
-        cx<T> acc = s[lpad(j)];
-        int e = 0;
-        for (int p = 1; p < R; ++p) {
-          e += qq;
-          if (e >= R) e -= R;
-          acc = cadd(acc, cmul(cmul(s[lpad(j + p * nb)], w[p - 1]), gld<cx<T>>(roots + e)));
-        }
-        d[lpad(ob + qq * Ns)] = acc;
-      }
     }
   }
 }
@@ -280,7 +323,8 @@ PF5D_DEV cx<T>* fft_lines(cx<T>* buf0, cx<T>* buf1, const LinePlanDev& pl, int L
   int Ns = 1;
   for (int p = 0; p < pl.npass; ++p) {
     const int R = pl.radix[p];
-    stockham_pass<T, NT>(a, b, pl.n, LS, G, R, Ns, tw + (pl.tw_off[p] - pl.tw_off[0]), roots + pl.root_off[p]);
+    stockham_pass<T, NT>(a, b, pl.n, LS, G, R, Ns, tw + (pl.tw_off[p] - pl.tw_off[0]),
+                         roots + (pl.root_off[p] - pl.root_off[0]));
     __syncthreads();
     cx<T>* t = a;
     a = b;
@@ -544,7 +588,7 @@ PF5D_DEV void gather_lds(int tot, int tid, cx<T>* dst, F fetch) {
   }
 }
 
-// LDS: the plan's twiddles (n slots), then two buffers of GL lines
+// LDS: the plan's twiddles (n slots) and roots (nroot slots), then two buffers of GL lines
 template <typename T, int NT>
 PF5D_DEV void fft_item_rt(const FftItem& I, const LinePlanDev& pl, const cx<T>* twl, const cx<T>* __restrict__ roots,
                           cx<T>* buf0, int tid) {
@@ -626,12 +670,14 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     if (I.plan != staged) {
-      // the previous item's buffers are free: every item ends in a barrier
+      // the previous item's buffers are free: every item ends in a barrier.
+      // LDS head: n - 1 twiddles, one spare slot, the generic radices' roots
       for (int i = tid; i < pl.n - 1; i += NT) twl[i] = gld<cx<T>>(tw + pl.tw_off[0] + i);
+      for (int i = tid; i < pl.nroot; i += NT) twl[pl.n + i] = gld<cx<T>>(roots + pl.root_off[0] + i);
       staged = I.plan;
       __syncthreads();
     }
-    cx<T>* buf = twl + pl.n;
+    cx<T>* buf = twl + pl.n + pl.nroot;
     switch (pl.n) {
       case 64: fft_item_ct<T, NT, 64>(I, twl, buf, tid); break;
       case 96: fft_item_ct<T, NT, 96>(I, twl, buf, tid); break;
@@ -639,7 +685,7 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
       case 192: fft_item_ct<T, NT, 192>(I, twl, buf, tid); break;
       case 256: fft_item_ct<T, NT, 256>(I, twl, buf, tid); break;
       case 512: fft_item_ct<T, NT, 512>(I, twl, buf, tid); break;
-      default: fft_item_rt<T, NT>(I, pl, twl, roots, buf, tid); break;
+      default: fft_item_rt<T, NT>(I, pl, twl, twl + pl.n, buf, tid); break;
     }
   }
 }

@@ -93,8 +93,10 @@ def test_scalar_kernels(dev):
     assert out_of_bounds(bad)
 
 
-@pytest.mark.parametrize("dim,shape", [(1, (160,)), (1, (189,)), (2, (48, 96)), (2, (64, 64)), (3, (16, 24, 32)),
-                                       (3, (12, 12, 12))])
+# lengths with large prime factors (101, 193; 7 x 11 x 13) run the direct-DFT passes
+@pytest.mark.parametrize("dim,shape", [(1, (160,)), (1, (189,)), (1, (808,)), (1, (2316,)), (1, (1001,)), (2, (48, 96)),
+                                       (2, (64, 64)), (2, (52, 22)), (3, (16, 24, 32)), (3, (12, 12, 12)),
+                                       (3, (14, 22, 26))])
 def test_elastic_update(dev, dim, shape):
     from phasefield5d_amd.solver.elastic import make_elastic_updater
     from phasefield5d_amd.solver.system import (build_fourier_grid_1d_along_direction,
@@ -132,12 +134,19 @@ def _split_variant(eng):
     return next(i for i, v in enumerate(eng.variants()) if v[0] == 0)
 
 
+def _fused_variant(eng):
+    """Index of the first single-pass stencil tiling (by > 0)."""
+    return next(i for i, v in enumerate(eng.variants()) if v[0] > 0)
+
+
 def _engine_one_step(pb, precision="fp64", form="fused"):
     from phasefield5d_amd.engine import Engine
     eng = Engine(precision)
     slot = eng.add_problem(pb)
     if form == "split":
         eng.set_tiling(_split_variant(eng), 1)
+    else:
+        eng.set_tiling(_fused_variant(eng), 16)
     eng.step(1)
     eng.sync()
     return eng, slot
