@@ -252,6 +252,11 @@ struct FftPlans {
     }
     pl.nroot = (int)roots.size() - pl.root_off[0];
     require(pl.n + pl.nroot == fft_head_slots(n), "FFT plan roots do not match the LDS layout");
+    // a line whose twiddles, roots and two buffers exceed a block's LDS reads
+    // twiddles and roots from global memory (1D lines of a few thousand points)
+    pl.tw_global = ((size_t)fft_head_slots(n) + 2 * (size_t)line_stride(n)) * sizeof(cx<T>) > (size_t)devinfo().lds;
+    require(2 * (size_t)line_stride(n) * sizeof(cx<T>) <= (size_t)devinfo().lds,
+            "FFT line too long for LDS: " + std::to_string(n) + " points");
     int id = (int)plans.size();
     plans.push_back(pl);
     index[n] = id;
@@ -360,8 +365,14 @@ struct FftBatch {
   static bool paired(int kind) { return kind == FK_R2C || kind == FK_C2R; }
   // complex lines per item: two line buffers within the LDS budget (real-line
   // stages carry two lines per complex line)
+  // LDS slots ahead of the line buffers of length n's items (none when the
+  // plan keeps its twiddles in global memory)
+  size_t head_slots(int n) const {
+    const auto it = plans.index.find(n);
+    return (it != plans.index.end() && plans.plans[it->second].tw_global) ? 0 : (size_t)fft_head_slots(n);
+  }
   int pick_g(int n, long nlines, int kind) const {
-    const size_t head = (size_t)fft_head_slots(n) * sizeof(cx<T>);
+    const size_t head = head_slots(n) * sizeof(cx<T>);
     const size_t avail = budget > head ? budget - head : 0;
     long g = (long)(avail / (2 * (size_t)line_stride(n) * sizeof(cx<T>)));
     g = std::max<long>(1, std::min<long>(g, 128));
@@ -371,7 +382,7 @@ struct FftBatch {
   void push(int st, FftItem it, int n) {
     const size_t gl = paired(it.kind) ? (it.G + 1) / 2 : it.G;
     stage[st].push_back(it);
-    lds[st] = std::max(lds[st], ((size_t)fft_head_slots(n) + 2 * gl * line_stride(n)) * sizeof(cx<T>));
+    lds[st] = std::max(lds[st], (head_slots(n) + 2 * gl * line_stride(n)) * sizeof(cx<T>));
   }
   // lines [0, nlines) with line l at base + l*ls; split into groups of G (a
   // multiple of align lines when G >= align)

@@ -64,6 +64,8 @@ struct LinePlanDev {
   int tw_off[FFT_MAXPASS];    // twiddles of pass p: tw[tw_off + k*(R-1) + (r-1)], k < Ns, 1 <= r < R
   int root_off[FFT_MAXPASS];  // roots exp(-2 pi i m / R), m < R (generic radices only)
   int nroot;                  // roots of all generic passes (contiguous from root_off[0])
+  int tw_global;              // twiddles and roots stay in global memory (lines too long for
+                              // them and two line buffers in LDS)
 };
 
 enum FftKind { FK_R2C = 0, FK_FWD = 1, FK_CONV = 2, FK_INV = 3, FK_C2R = 4, FK_LINE = 5 };
@@ -669,15 +671,18 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
     // being hoisted out of the item loop (it would hold registers across items)
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
+    const bool tg = pl.tw_global != 0;
     if (I.plan != staged) {
       // the previous item's buffers are free: every item ends in a barrier.
       // LDS head: n - 1 twiddles, one spare slot, the generic radices' roots
-      for (int i = tid; i < pl.n - 1; i += NT) twl[i] = gld<cx<T>>(tw + pl.tw_off[0] + i);
-      for (int i = tid; i < pl.nroot; i += NT) twl[pl.n + i] = gld<cx<T>>(roots + pl.root_off[0] + i);
+      if (!tg) {
+        for (int i = tid; i < pl.n - 1; i += NT) twl[i] = gld<cx<T>>(tw + pl.tw_off[0] + i);
+        for (int i = tid; i < pl.nroot; i += NT) twl[pl.n + i] = gld<cx<T>>(roots + pl.root_off[0] + i);
+      }
       staged = I.plan;
       __syncthreads();
     }
-    cx<T>* buf = twl + pl.n + pl.nroot;
+    cx<T>* buf = tg ? twl : twl + pl.n + pl.nroot;
     switch (pl.n) {
       case 64: fft_item_ct<T, NT, 64>(I, twl, buf, tid); break;
       case 96: fft_item_ct<T, NT, 96>(I, twl, buf, tid); break;
@@ -685,7 +690,12 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
       case 192: fft_item_ct<T, NT, 192>(I, twl, buf, tid); break;
       case 256: fft_item_ct<T, NT, 256>(I, twl, buf, tid); break;
       case 512: fft_item_ct<T, NT, 512>(I, twl, buf, tid); break;
-      default: fft_item_rt<T, NT>(I, pl, twl, twl + pl.n, buf, tid); break;
+      default:
+        if (tg)
+          fft_item_rt<T, NT>(I, pl, tw + pl.tw_off[0], roots + pl.root_off[0], buf, tid);
+        else
+          fft_item_rt<T, NT>(I, pl, twl, twl + pl.n, buf, tid);
+        break;
     }
   }
 }

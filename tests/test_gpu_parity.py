@@ -93,8 +93,11 @@ def test_scalar_kernels(dev):
     assert out_of_bounds(bad)
 
 
-# lengths with large prime factors (101, 193; 7 x 11 x 13) run the direct-DFT passes
-@pytest.mark.parametrize("dim,shape", [(1, (160,)), (1, (189,)), (1, (808,)), (1, (2316,)), (1, (1001,)), (2, (48, 96)),
+# lengths with large prime factors (101, 193; 7 x 11 x 13) run the direct-DFT passes;
+# FP64 lines of 3200 (the longest 1D request) and 4002 = 2 x 3 x 23 x 29 keep their
+# twiddles in global memory
+@pytest.mark.parametrize("dim,shape", [(1, (160,)), (1, (189,)), (1, (808,)), (1, (2316,)), (1, (1001,)), (1, (3200,)),
+                                       (1, (4002,)), (2, (48, 96)),
                                        (2, (64, 64)), (2, (52, 22)), (3, (16, 24, 32)), (3, (12, 12, 12)),
                                        (3, (14, 22, 26))])
 def test_elastic_update(dev, dim, shape):
@@ -153,11 +156,18 @@ def _engine_one_step(pb, precision="fp64", form="fused"):
 
 
 @pytest.mark.parametrize("form", ["fused", "split"])
-@pytest.mark.parametrize("dim,n,elastic", [(1, 160, True), (2, 32, True), (2, 32, False), (3, 16, True),
-                                           (3, 24, False)])
+@pytest.mark.parametrize("dim,n,elastic", [(1, 160, True), (1, 3200, True), (2, 32, True), (2, 32, False),
+                                           (3, 16, True), (3, 24, False)])
 def test_fused_step_fp64(dev, dim, n, elastic, form):
     pb = problem(dim, n, elastic=elastic)
-    pb.X_init = developed_state(pb)
+    if n == 3200:
+        # the longest 1D request (its FFT keeps twiddles in global memory in FP64),
+        # with the 160-cell field's wavelengths: the flux stencil differences psi
+        # twice, so FFT rounding relative to dX grows as (wavelength / 2 pi)^2 and a
+        # field of 1000-cell modes would measure that, not the step
+        pb.X_init = np.tile(developed_state(problem(1, 160, elastic=elastic)), (20, 1))
+    else:
+        pb.X_init = developed_state(pb)
     ora = oracle_sim(pb)
     Xn, dX, dt_new, _, xmax = ora.step(pb.X_init, pb.dt0)
     eng, slot = _engine_one_step(pb, form=form)
