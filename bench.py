@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--calibration-copy", action="store_true",
                     help="end with a 256 MiB device-to-device copy (scales FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--fp32-only", action="store_true", help="only the FP32 fused leg (profiling runs)")
+    ap.add_argument("--no-developed", action="store_true",
+                    help="skip the developed-field FP32 leg (profiling runs of the headline workload)")
     ap.add_argument("--no-mix", action="store_true", help="skip the 64-run batched mix leg")
     ap.add_argument("--mix-steps", type=int, default=20)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
@@ -187,22 +189,24 @@ def main():
     # the same step on a decomposed-looking field (neighbouring cells in different
     # table hypercubes, some past the linear region), as later in a run
     from phasefield5d_amd.workloads import developed_field
-    ed = Engine("fp32")
-    pbd = dataclasses.replace(pb, X_init=developed_field(pb, amp=0.05, seed=4))
-    sd_ = ed.add_problem(pbd)
-    ed.set_tiling(info["variant"], info["cx"])
-    ed.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
-    ed.step(5)
-    ed.sync()
-    t0 = time.perf_counter()
-    ed.step(50)
-    ed.sync()
-    eld = time.perf_counter() - t0
-    developed = {"amplitude": 0.05, "steps": 50, "ms_per_step": 1e3 * eld / 50, "cell_updates_per_s": ncell * 50 / eld,
-                 "k1_potentials_ms": ed.time_kernel("k1_potentials", reps=10) if split else None,
-                 "active": bool(ed.state(sd_)["status"] & 1)}
-    ed.close()
-    log("fp32 developed field", developed)
+    developed = None
+    ed = None if args.no_developed else Engine("fp32")
+    if ed is not None:
+        pbd = dataclasses.replace(pb, X_init=developed_field(pb, amp=0.05, seed=4))
+        sd_ = ed.add_problem(pbd)
+        ed.set_tiling(info["variant"], info["cx"])
+        ed.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
+        ed.step(5)
+        ed.sync()
+        t0 = time.perf_counter()
+        ed.step(50)
+        ed.sync()
+        eld = time.perf_counter() - t0
+        developed = {"amplitude": 0.05, "steps": 50, "ms_per_step": 1e3 * eld / 50, "cell_updates_per_s": ncell * 50 / eld,
+                     "k1_potentials_ms": ed.time_kernel("k1_potentials", reps=10) if split else None,
+                     "active": bool(ed.state(sd_)["status"] & 1)}
+        ed.close()
+        log("fp32 developed field", developed)
 
     mass0 = pb.X_init.reshape(-1, 4).sum(axis=0)
     drift32 = np.abs(st32["mass"] - mass0) / mass0

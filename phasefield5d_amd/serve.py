@@ -141,11 +141,11 @@ class Server:
                 if self.used_bytes + need > self.hbm_budget:
                     break
                 ready.pop(0)
-                req.slot = self.engine.add_problem(req.problem, t_end=req.t_end)
                 ta = time.perf_counter()
-                req.admitted = ta - t0
+                req.slot = self.engine.add_problem(req.problem, t_end=req.t_end)
+                req.admitted = time.perf_counter() - t0
                 req.extra["bytes"] = need
-                req.extra["admit_s"] = time.perf_counter() - ta
+                req.extra["admit_s"] = time.perf_counter() - ta  # table, B-hat and field on the device
                 self.used_bytes += need
                 running[req.slot] = req
                 admitted = True
