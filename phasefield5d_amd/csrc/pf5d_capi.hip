@@ -1220,6 +1220,7 @@ struct Engine : EngineBase {
     }
     for (auto& s : sims) s.fresh = false;
     dirty = false;
+    stats_stale = false;
   }
   void launch_k2(int prime) {
     if (!nk2) return;
@@ -1295,10 +1296,24 @@ struct Engine : EngineBase {
       n -= GN;
     }
     while (n-- > 0) HIPCHK(hipGraphLaunch(g1, stream));
+    stats_stale = true;
+  }
+  // steps compute the FP64 statistics only ahead of bounds checks: bring them
+  // up to date (statistics of the current fields, nothing advanced) before a
+  // state is read
+  bool stats_stale = false;
+  void refresh_stats() {
+    if (!stats_stale) return;
+    if (nk2) {
+      launch_k2(1);
+      launch_dt(0);
+    }
+    stats_stale = false;
   }
   void get_state(int slot, double* out) override {
     at(slot);
     if (dirty) rebuild();
+    refresh_stats();
     SimState st;
     HIPCHK(hipMemcpyAsync(&st, d_state + slot, sizeof(SimState), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -1309,6 +1324,7 @@ struct Engine : EngineBase {
   // all slots in one copy: out[16*i ...] for slot i < nslots (unused slots: status 0)
   int get_all_states(int maxn, double* out) override {
     if (dirty) rebuild();
+    refresh_stats();
     const int n = std::min(maxn, (int)sims.size());
     if (n <= 0) return 0;
     std::vector<SimState> h(n);

@@ -445,8 +445,10 @@ PF5D_DEV void dt_control(int sim, const SimDev<T>* __restrict__ sims, SimState* 
     // atomic exchange: the maxima were folded in with device-scope atomics
     const double x = (double)__hip_atomic_exchange(&xmax_acc[sim], T(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     [&] {
-      if (!(S.status & ST_ACTIVE)) return;
-      if (p1 > p0) {
+      // statistics of the fields as the last update (or a prime launch) left
+      // them: kept for inactive simulations only from a prime launch (advance
+      // 0), which computes them from the current fields
+      if (p1 > p0 && (!advance || (S.status & ST_ACTIVE))) {
         for (int a = 0; a < 4; ++a) {
           S.mass[a] = v[a];
           S.mean[a] = v[a] / (double)sd.ncell;
@@ -457,7 +459,7 @@ PF5D_DEV void dt_control(int sim, const SimDev<T>* __restrict__ sims, SimState* 
         S.cmin = v[7];
         S.cmax = v[8];
       }
-      if (!advance) return;
+      if (!(S.status & ST_ACTIVE) || !advance) return;
       // bounds check of X_n before step n (the reference checks every 100 steps)
       if (sd.oob_interval > 0 && (S.steps % sd.oob_interval) == 0) {
         const double th = sd.tol_hard, ts = sd.tol_soft;
@@ -519,6 +521,11 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
     const bool run = prime || (S.status & ST_ACTIVE);
     const SimDev<T>& sd = sims[item.sim];
     const double dt = S.dt;
+    // the FP64 statistics (mass, bounds) are read on the device only by the
+    // bounds check of the next step, every oob_interval steps (S.steps already
+    // counts this step); the host refreshes them with a prime launch before it
+    // reads a state, so other steps skip them
+    const bool stats = prime || (sd.oob_interval > 0 && (S.steps % sd.oob_interval) == 0);
     double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
     double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
     T ref[4];
@@ -579,21 +586,24 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
         for (int a = 1; a < 4; ++a) acc = mad<T>((T)(xd[a] - (double)ref[a]), sd.l[a], acc);
         gst<T>(sd.s + c, acc);
       }
-      double csum = 0;
-      bool fin = true;
+      if (stats) {
+        double csum = 0;
+        bool fin = true;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const double v = xd[a];
-        neumaier_add<T>(sum[a], cmp[a], v);
-        fin = fin && isfinite(v);
-        csum += v;
-        cmin = fmin(cmin, v);
-        cmax = fmax(cmax, v);
+        for (int a = 0; a < 4; ++a) {
+          const double v = xd[a];
+          neumaier_add<T>(sum[a], cmp[a], v);
+          fin = fin && isfinite(v);
+          csum += v;
+          cmin = fmin(cmin, v);
+          cmax = fmax(cmax, v);
+        }
+        if (!fin) nf += 1;
+        smax = fmax(smax, csum);
+        smin = fmin(smin, csum);
       }
-      if (!fin) nf += 1;
-      smax = fmax(smax, csum);
-      smin = fmin(smin, csum);
     }
+    if (!stats) continue;  // uniform over the block: the item's simulation
     double v[NSTAT] = {sum[0] + cmp[0], sum[1] + cmp[1], sum[2] + cmp[2], sum[3] + cmp[3], nf, smax, smin, cmin, cmax};
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

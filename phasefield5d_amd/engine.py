@@ -40,10 +40,12 @@ class Engine:
         self.close()
 
     # -- membership ------------------------------------------------------------
-    def add_problem(self, pb, X=None, t_end=math.inf, oob_interval=100, dt0=None):
+    def add_problem(self, pb, X=None, t_end=math.inf, oob_interval=100, dt0=None, tol_soft=1e-5, tol_hard=1e-3):
         """Add a set-up problem (see problem.build_problem); returns its slot.  A problem
         without a host field (X_init None) and with ``extra["device_init"] = (seed,
-        fluctuation)`` has its initial field drawn on the device."""
+        fluctuation)`` has its initial field drawn on the device.  Every ``oob_interval``
+        steps the field is checked with the tolerances of ``out_of_bounds``
+        (system.py:12-53): past ``tol_hard`` the simulation stops."""
         cfg = pb.cfg
         d = _lib.SimDesc()
         d.dim = len(pb.shape)
@@ -60,7 +62,7 @@ class Engine:
         d.lower_limit, d.upper_limit = cfg.lower_limit, cfg.upper_limit
         d.t_end = t_end if math.isfinite(t_end) else 1e300
         d.oob_interval = oob_interval
-        d.tol_soft, d.tol_hard = 1e-5, 1e-3
+        d.tol_soft, d.tol_hard = float(tol_soft), float(tol_hard)
         bh = np.ascontiguousarray(pb.Bhat, dtype=np.float64) if pb.Bhat is not None else None
         tab = device_table(pb.data_grid, cfg.resolution, self.prec)
         slot = C.c_int()
