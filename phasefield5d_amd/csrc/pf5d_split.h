@@ -14,10 +14,12 @@
 // node box), so many more waves stay resident.  Which is faster depends on
 // the shape mix; the autotuner times both.
 //
-// Work items are (sim, x plane, y0, z0, ty, tz) tiles of at most NT cells:
-// thread t handles cell (x, y0 + t / tz, z0 + t % tz).  Items are listed with
-// x fastest and every block takes a contiguous range of them, so a block walks
-// consecutive planes of one tile and finds its x neighbours in cache.  A face flux is
+// Work items are (sim, x planes xa..xb-1, y0, z0, ty, tz) tiles of at most NT
+// cells per plane: thread t handles cells (x, y0 + t / tz, z0 + t % tz) for x
+// in [xa, xb), all served by one staged node box (KP; the per-plane flux pass
+// takes single-plane items).  Items are listed with x fastest and every block
+// takes a contiguous range of them, so a block walks consecutive planes of one
+// tile and finds its x neighbours in cache.  A face flux is
 // evaluated by both cells it separates, always with the lower cell first, so
 // both see the same value (the divergence telescopes exactly, as in the
 // single-pass form where each face is evaluated once).
@@ -65,15 +67,17 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
     const bool hx = sd.dim == 3, hy = sd.dim >= 2;
     int gx, gy, gz;
     const bool valid = tile_cell<T>(sd, item, tid, frcp(item.tz), gx, gy, gz);
-    const long c = ((long)gx * n1 + gy) * n2 + gz;
+    long c = ((long)gx * n1 + gy) * n2 + gz;
     V4<T> xc{};
     if (valid) xc = ld4g<T>(sd.X + 4 * c);
-    // node box around the tile's centre cell (the anchor): lower nodes one below its
-    // lower cube corners, so corners within +-1 of the anchor's are covered
+    // node box around the centre cell of the item's planes (the anchor): lower
+    // nodes one below its lower cube corners, so corners within +-1 of the
+    // anchor's are covered
     const int num = sd.tab.num;
     int lo[4];
     {
-      const long ca = ((long)item.xa * n1 + item.y0 + item.ty / 2) * n2 + item.z0 + item.tz / 2;
+      const long ca = ((long)(item.xa + (item.xb - 1 - item.xa) / 2) * n1 + item.y0 + item.ty / 2) * n2 + item.z0 +
+                      item.tz / 2;
       const V4<T> xa = ld4g<T>(sd.X + 4 * ca);
       // (the box position only decides which points read LDS: clamped in
       // floating point first, so NaN lands on 0 and the conversion is in range)
@@ -100,16 +104,19 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
       }
     }
     __syncthreads();
-    if (valid) {
+    // the item's planes xa .. xb-1 share the staged box: one cell per plane
+    const long sx = (long)n1 * n2;
+#pragma unroll 1
+    for (gx = item.xa; valid && gx < item.xb; ++gx, c += sx) {
+      if (gx > item.xa) xc = ld4g<T>(sd.X + 4 * c);
       T P[9];
       interp9_box4<T>(sd.tab, BOX, lo, xc.v, P);
       // neighbour sums in the reference order: x, then y, then z (each up + down)
       T acc[4];
       bool first = true;
       if (hx) {
-        const long s = (long)n1 * n2;
-        const V4<T> u = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? s : s - (long)n0 * s)));
-        const V4<T> d = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -s : (long)(n0 - 1) * s)));
+        const V4<T> u = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx)));
+        const V4<T> d = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -sx : (long)(n0 - 1) * sx)));
 #pragma unroll
         for (int a = 0; a < 4; ++a) acc[a] = u.v[a] + d.v[a];
         first = false;

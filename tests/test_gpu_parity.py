@@ -382,14 +382,15 @@ def test_stencil_forms_bitwise_uneven_shapes(dev, precision):
     edges: no more rows than a block has wavefronts (the periodic y wrap inside a
     tile; 4 cells is the engine's minimum per axis), a row just longer than a
     tile (a 2-cell tail), odd row lengths (a lane
-    holding the row's last cell in its first slot), a short 2D plane, a 1D line
-    of odd length (elasticity off: the stencil alone)."""
+    holding the row's last cell in its first slot), 37 planes (slabs of 4 or 5
+    planes, so potentials items of several planes end short), a short 2D plane,
+    a 1D line of odd length (elasticity off: the stencil alone)."""
     import dataclasses
     from phasefield5d_amd.engine import Engine
     base = {3: problem(3, 8, seed=21, elastic=False), 2: problem(2, 8, seed=22, elastic=False),
             1: problem(1, 8, seed=23, elastic=False)}
     pbs = []
-    for shape in [(6, 4, 130), (5, 4, 33), (4, 9, 65), (7, 200), (129,)]:
+    for shape in [(6, 4, 130), (5, 4, 33), (4, 9, 65), (37, 6, 20), (7, 200), (129,)]:
         p = dataclasses.replace(base[len(shape)], shape=tuple(shape), Bhat=None)
         p.X_init = developed_state(p, amp=0.03, seed=len(pbs) + 3)
         pbs.append(p)
