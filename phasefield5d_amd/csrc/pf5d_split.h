@@ -104,21 +104,34 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
       }
     }
     __syncthreads();
-    // the item's planes xa .. xb-1 share the staged box: one cell per plane
+    // the item's planes xa .. xb-1 share the staged box: one cell per plane.
+    // FP32 walks up in x with the cell below (xm) and the cell itself carried
+    // over from the previous plane; FP64 reloads them (carrying them costs it
+    // registers the fallback paths need)
+    constexpr bool carry = sizeof(T) == 4;
     const long sx = (long)n1 * n2;
+    V4<T> xm{};
+    if (carry && valid && hx) xm = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -sx : (long)(n0 - 1) * sx)));
 #pragma unroll 1
     for (gx = item.xa; valid && gx < item.xb; ++gx, c += sx) {
-      if (gx > item.xa) xc = ld4g<T>(sd.X + 4 * c);
+      if (!carry && gx > item.xa) xc = ld4g<T>(sd.X + 4 * c);
+      V4<T> xp{};
+      if (carry && hx) xp = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx)));
       T P[9];
       interp9_box4<T>(sd.tab, BOX, lo, xc.v, P);
       // neighbour sums in the reference order: x, then y, then z (each up + down)
       T acc[4];
       bool first = true;
       if (hx) {
-        const V4<T> u = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx)));
-        const V4<T> d = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -sx : (long)(n0 - 1) * sx)));
+        if constexpr (carry) {
 #pragma unroll
-        for (int a = 0; a < 4; ++a) acc[a] = u.v[a] + d.v[a];
+          for (int a = 0; a < 4; ++a) acc[a] = xp.v[a] + xm.v[a];
+        } else {
+          const V4<T> u = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx)));
+          const V4<T> d = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -sx : (long)(n0 - 1) * sx)));
+#pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a] = u.v[a] + d.v[a];
+        }
         first = false;
       }
       if (hy) {
@@ -149,6 +162,10 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
       gst_field<T, V4<T>>(sd.mu + 4 * c, mu);
       gst_field<T, V4<T>>(sd.mb + 4 * c, mb);
       gst_field<T, T>(sd.mb1 + c, P[8]);
+      if (carry) {
+        xm = xc;
+        xc = xp;
+      }
     }
     __syncthreads();  // BOX is reused by the next item
   }
