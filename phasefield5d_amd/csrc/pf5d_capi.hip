@@ -798,7 +798,7 @@ std::vector<K1Variant<float>> k1_variants<float>() {
           make_split_variant<float, 256, 4, 1>(), make_split_variant<float, 256, 4, 1, 32>(),
           make_split_variant<float, 256, 4, 2>(), make_march_variant<float, 256, 4, 256, 1>(),
           make_march_variant<float, 256, 4, 256, 2>(), make_march_variant<float, 256, 4, 512, 2>(),
-          make_march_variant<float, 256, 4, 256, 2, 3>(), make_march_variant<float, 256, 4, 256, 2, 3, 64>()};
+          make_march_variant<float, 256, 4, 256, 2, 3>()};
 }
 
 constexpr int K2_NT = 256;
