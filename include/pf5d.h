@@ -182,6 +182,9 @@ int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes);
  * FFT items, K1 grid, K1 LDS bytes, stencil variant, x chunk, FFT LDS budget bytes, FFT block
  * size, flux-pass grid, FFT plane parts in use (0: line passes only), FFT plane block size] */
 int pf5d_engine_launch_info(pf5d_engine* e, long out[12]);
+/* number of simulation groups the engine steps one after another (each group's per-step
+ * working set fits the GPU's last-level cache; 1: all simulations step together) */
+int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups);
 
 /* average device time of one launch of a stage (HIP events on the engine stream):
  * which = 0 stencil stage K1 (either form), 1 the FFT passes, 2 dt controller, 3 update

@@ -730,6 +730,7 @@ struct EngineBase {
   virtual void set_fft_planes(int parts, int nt) = 0;
   virtual int num_variants(int* info) = 0;
   virtual void launch_info(long* out) = 0;
+  virtual int num_groups() = 0;
   virtual double time_kernel(int which, int reps) = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -752,7 +753,7 @@ struct K1Variant {
   // 64*fcpl cells, fnt/64 rows per block (fmarch, which can also run the dt
   // controller in its last block)
   int fcpl = 0, fnt = 0;
-  void (*fmarch)(const SimDev<T>*, SimState*, T*, const K1Item*, int, const double*, const int*, int,
+  void (*fmarch)(const SimDev<T>*, SimState*, T*, const K1Item*, int, const double*, const int*, int, const int*,
                  unsigned*) = nullptr;
   int fminw = 0;  // marching pass: min waves per SIMD it was compiled for
   bool split() const { return by == 0; }
@@ -844,6 +845,35 @@ struct Engine : EngineBase {
   hipGraphExec_t g1 = nullptr, gN = nullptr;
   static constexpr int GN = 8;
   unsigned k1_grid = 0, kf_grid = 0;
+  // Simulations whose per-step working sets together exceed what the 256 MB
+  // Infinity Cache holds are stepped in groups: each group runs GN steps (its
+  // own graph) while its fields stay in the cache, then the next group.  Every
+  // simulation advances by the same steps; only the order of independent work
+  // changes.  A group is a range of the work lists, its own FFT batch and the
+  // list of its simulations for the dt controller.  At most one group: the
+  // whole engine steps as one graph.
+  struct Group {
+    std::vector<int> sims;
+    int k1_off = 0, nk1 = 0, kf_off = 0, nkf = 0, k2_off = 0, nk2 = 0;
+    unsigned k1_grid = 0, kf_grid = 0, k2_grid = 0;
+    int* d_list = nullptr;
+    FftBatch<T> fft;
+    hipGraphExec_t g1 = nullptr, gN = nullptr;
+    void drop() {
+      if (g1) (void)hipGraphExecDestroy(g1);
+      if (gN) (void)hipGraphExecDestroy(gN);
+      g1 = gN = nullptr;
+    }
+    ~Group() {
+      drop();
+      dfree(d_list);
+    }
+  };
+  std::vector<std::unique_ptr<Group>> groups;
+  // per-step working set, bytes per cell: X (+ remainder), dX, mu, Mob, s, psi
+  // and the spectrum; a group holds at most GROUP_BYTES of it
+  static constexpr size_t STEP_BYTES = sizeof(T) == 4 ? 100 : 170;
+  static constexpr size_t GROUP_BYTES = size_t(192) << 20;
 
   Engine(hipStream_t s) {
     if (s) {
@@ -917,6 +947,7 @@ struct Engine : EngineBase {
     if (g1) (void)hipGraphExecDestroy(g1);
     if (gN) (void)hipGraphExecDestroy(gN);
     g1 = gN = nullptr;
+    for (auto& g : groups) g->drop();
   }
   void ensure_capacity(int n) {
     if (n <= cap) return;
@@ -1117,9 +1148,36 @@ struct Engine : EngineBase {
     std::vector<K2Item> k2;
     std::vector<int> prange(2 * cap, 0);
     fft.clear();
-    for (int i = 0; i < (int)sims.size(); ++i) {
+    groups.clear();
+    // groups (marching two-pass form only): consecutive simulations up to
+    // GROUP_BYTES of working set, at least one each
+    std::vector<std::vector<int>> gsims;
+    {
+      const bool can_group = V.split() && V.fcpl > 0;
+      size_t acc = 0;
+      for (int i = 0; i < (int)sims.size(); ++i) {
+        if (!sims[i].used) continue;
+        const size_t b = (size_t)sims[i].ncell * STEP_BYTES;
+        if (gsims.empty() || (can_group && !gsims.back().empty() && acc + b > GROUP_BYTES)) {
+          gsims.emplace_back();
+          acc = 0;
+        }
+        gsims.back().push_back(i);
+        acc += b;
+      }
+    }
+    std::vector<int> order, gstart;  // simulations in group order; first of each group in order
+    for (auto& g : gsims) {
+      gstart.push_back((int)order.size());
+      order.insert(order.end(), g.begin(), g.end());
+    }
+    std::vector<int> k1_at(order.size() + 1), kf_at(order.size() + 1), k2_at(order.size() + 1);
+    for (int oi = 0; oi < (int)order.size(); ++oi) {
+      const int i = order[oi];
       Sim& s = sims[i];
-      if (!s.used) continue;
+      k1_at[oi] = (int)k1.size();
+      kf_at[oi] = (int)kf.size();
+      k2_at[oi] = (int)k2.size();
       devs[i] = make_dev(s);
       const SimDev<T>& v = devs[i];
       const bool hx = v.dim == 3;
@@ -1177,8 +1235,37 @@ struct Engine : EngineBase {
       if (s.d.elastic)
         fft.add_sim(i, s.d.dim, s.d.shape, s.s, s.C, s.psi, s.bhat);
     }
+    k1_at[order.size()] = (int)k1.size();
+    kf_at[order.size()] = (int)kf.size();
+    k2_at[order.size()] = (int)k2.size();
     fft.plans.upload();
     fft.upload();
+    if (gsims.size() > 1) {
+      for (size_t gi = 0; gi < gsims.size(); ++gi) {
+        auto g = std::make_unique<Group>();
+        g->sims = gsims[gi];
+        const int a = gstart[gi], b = a + (int)g->sims.size();
+        g->k1_off = k1_at[a];
+        g->nk1 = k1_at[b] - k1_at[a];
+        g->kf_off = kf_at[a];
+        g->nkf = kf_at[b] - kf_at[a];
+        g->k2_off = k2_at[a];
+        g->nk2 = k2_at[b] - k2_at[a];
+        g->fft.budget = fft.budget;
+        g->fft.nt = fft.nt;
+        g->fft.parts = fft.parts;
+        g->fft.pnt = fft.pnt;
+        for (int i : g->sims) {
+          const Sim& s = sims[i];
+          if (s.d.elastic) g->fft.add_sim(i, s.d.dim, s.d.shape, s.s, s.C, s.psi, s.bhat);
+        }
+        g->fft.plans.upload();
+        g->fft.upload();
+        g->d_list = (int*)dmalloc(g->sims.size() * sizeof(int));
+        HIPCHK(hipMemcpy(g->d_list, g->sims.data(), g->sims.size() * sizeof(int), hipMemcpyHostToDevice));
+        groups.push_back(std::move(g));
+      }
+    }
     HIPCHK(hipMemcpy(d_sims, devs.data(), cap * sizeof(SimDev<T>), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_prange, prange.data(), prange.size() * sizeof(int), hipMemcpyHostToDevice));
     dfree(d_k1);
@@ -1215,6 +1302,13 @@ struct Engine : EngineBase {
       const void* ff = V.fcpl > 0 ? (const void*)V.fmarch : (const void*)V.fflux;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ff, V.fnt, 0));
       kf_grid = V.fcpl > 0 ? xcd_grid(std::min(nkf, devinfo().cus * std::max(1, occ))) : grid(occ);
+      int occ_pot = 1;
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_pot, (const void*)V.fpot, V.nt, 0));
+      for (auto& g : groups) {
+        g->k1_grid = xcd_grid(std::max(1, std::min(g->nk1, devinfo().cus * std::max(1, occ_pot))));
+        g->kf_grid = xcd_grid(std::max(1, std::min(g->nkf, devinfo().cus * std::max(1, occ))));
+        g->k2_grid = xcd_grid(std::max(1, std::min(g->nk2, devinfo().cus * 8)));
+      }
     } else {
       (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
       occ = std::max(1, occ);
@@ -1240,7 +1334,7 @@ struct Engine : EngineBase {
   }
   void launch_dt(int advance) {
     hipLaunchKernelGGL((k_dtctl<T, DT_NT>), dim3((unsigned)sims.size()), dim3(DT_NT), 0, stream, d_sims, d_state, d_xmax,
-                       d_part, d_prange, (int)sims.size(), advance);
+                       d_part, d_prange, (int)sims.size(), advance, nullptr);
     HIPCHK(hipGetLastError());
   }
   // the flux pass; with_dt: the marching pass also runs the dt controller
@@ -1249,7 +1343,7 @@ struct Engine : EngineBase {
     if (V.fcpl > 0) {
       const int ndt = (with_dt && fuse_dt_ok()) ? (int)sims.size() : 0;
       hipLaunchKernelGGL(V.fmarch, dim3(kf_grid), dim3(V.fnt), 0, stream, d_sims, d_state, d_xmax, d_kf, nkf, d_part,
-                         d_prange, ndt, d_done);
+                         d_prange, ndt, nullptr, d_done);
       return ndt > 0;
     }
     hipLaunchKernelGGL(V.fflux, dim3(kf_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_xmax, d_k1, nk1);
@@ -1279,11 +1373,39 @@ struct Engine : EngineBase {
     if (!launch_k1(true)) launch_dt(1);
     launch_k2(0);
   }
-  hipGraphExec_t capture(int n) {
+  // one step of one group (marching two-pass form): its FFT passes, potentials,
+  // flux with the dt controller of its simulations, update
+  void enqueue_group_step(Group& g) {
+    const K1Variant<T>& V = variants[variant];
+    g.fft.run(stream, d_state);
+    if (g.nk1) {
+      hipLaunchKernelGGL(V.fpot, dim3(g.k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1 + g.k1_off, g.nk1);
+      HIPCHK(hipGetLastError());
+    }
+    const int ns = (int)g.sims.size();
+    const int ndt = (g.nkf && g.nk2 && ns <= 8) ? ns : 0;
+    if (g.nkf) {
+      hipLaunchKernelGGL(V.fmarch, dim3(g.kf_grid), dim3(V.fnt), 0, stream, d_sims, d_state, d_xmax, d_kf + g.kf_off,
+                         g.nkf, d_part, d_prange, ndt, g.d_list, d_done);
+      HIPCHK(hipGetLastError());
+    }
+    if (!ndt && g.nk2) {
+      hipLaunchKernelGGL((k_dtctl<T, DT_NT>), dim3((unsigned)ns), dim3(DT_NT), 0, stream, d_sims, d_state, d_xmax,
+                         d_part, d_prange, ns, 1, g.d_list);
+      HIPCHK(hipGetLastError());
+    }
+    if (g.nk2) {
+      hipLaunchKernelGGL((k_update<T, K2_NT>), dim3(g.k2_grid), dim3(K2_NT), 0, stream, d_sims, d_state,
+                         d_k2 + g.k2_off, g.nk2, d_part + (long)g.k2_off * NSTAT, 0);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  template <typename F>
+  hipGraphExec_t capture_with(int n, F enqueue) {
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (int i = 0; i < n; ++i) enqueue_step();
+      for (int i = 0; i < n; ++i) enqueue();
     } catch (...) {
       (void)hipStreamEndCapture(stream, &g);
       throw;
@@ -1294,10 +1416,32 @@ struct Engine : EngineBase {
     (void)hipGraphDestroy(g);
     return ex;
   }
+  hipGraphExec_t capture(int n) {
+    return capture_with(n, [&] { enqueue_step(); });
+  }
+  int num_groups() override {
+    if (dirty) rebuild();
+    return std::max<int>(1, (int)groups.size());
+  }
   void step(int n) override {
     require(n >= 0, "nsteps must be >= 0");
     if (dirty) rebuild();
     if (nk1 == 0 || n == 0) return;
+    if (groups.size() > 1) {
+      for (auto& gp : groups) {
+        Group& g = *gp;
+        if (!g.g1) g.g1 = capture_with(1, [&] { enqueue_group_step(g); });
+        if (n >= GN && !g.gN) g.gN = capture_with(GN, [&] { enqueue_group_step(g); });
+      }
+      while (n >= GN) {
+        for (auto& g : groups) HIPCHK(hipGraphLaunch(g->gN, stream));
+        n -= GN;
+      }
+      for (auto& g : groups)
+        for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(g->g1, stream));
+      stats_stale = true;
+      return;
+    }
     if (!g1) g1 = capture(1);
     if (n >= GN && !gN) gN = capture(GN);
     while (n >= GN) {
@@ -1907,6 +2051,10 @@ int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes) {
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms) {
   return guarded([&] { *avg_ms = e->impl->time_kernel(which, reps); });
 }
+int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups) {
+  return guarded([&] { *ngroups = e->impl->num_groups(); });
+}
+
 int pf5d_engine_launch_info(pf5d_engine* e, long out[12]) {
   return guarded([&] { e->impl->launch_info(out); });
 }

@@ -485,13 +485,14 @@ PF5D_DEV void dt_control(int sim, const SimDev<T>* __restrict__ sims, SimState* 
   __syncthreads();  // red is reused by the next call
 }
 
-// one block per simulation
+// one block per simulation: simulation list[blockIdx.x] (blockIdx.x without a list)
 template <typename T, int NTB>
 __global__ void __launch_bounds__(NTB) k_dtctl(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st,
                                                T* __restrict__ xmax_acc, const double* __restrict__ part,
-                                               const int* __restrict__ part_range, int nsims, int advance) {
-  const int sim = blockIdx.x;
-  if (sim >= nsims) return;
+                                               const int* __restrict__ part_range, int nsims, int advance,
+                                               const int* __restrict__ list) {
+  if ((int)blockIdx.x >= nsims) return;
+  const int sim = list ? list[blockIdx.x] : (int)blockIdx.x;
   __shared__ double red[NTB / 64][NSTAT];
   dt_control<T, NTB>(sim, sims, st, xmax_acc, part, part_range, advance, red, threadIdx.x);
 }

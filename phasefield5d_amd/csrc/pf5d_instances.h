@@ -38,7 +38,7 @@ namespace pf5d {
   PF5D_INST __global__ void k_fluxmarch<T, NT, CPL, MINW>(const SimDev<T>* __restrict__, SimState* __restrict__, \
                                                     T* __restrict__, const K1Item* __restrict__, int,       \
                                                     const double* __restrict__, const int* __restrict__, int, \
-                                                    unsigned* __restrict__);
+                                                    const int* __restrict__, unsigned* __restrict__);
 #define PF5D_FLUX_INSTANCE(T, NT, LOADS)                                                                       \
   PF5D_INST __global__ void k_fluxdiv<T, NT, LOADS>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
                                                     T* __restrict__, const K1Item* __restrict__, int);

@@ -405,7 +405,8 @@ PF5D_DEV CellV<T> shfl_cell_from(const CellV<T>& v, int src) {
 // and the divergence is summed in the reference order.
 //
 // With ndt > 0 the block that finishes last also runs the dt controller of
-// simulations 0 .. ndt-1 (the step's DT stage, without a launch of its own):
+// simulations dt_list[0 .. ndt-1] (0 .. ndt-1 without a list; the step's DT
+// stage, without a launch of its own):
 // every block counts itself in *done after its maxima are folded in; the last
 // one to count resets the counter for the next launch.
 // MINW: minimum waves per SIMD the register allocation must allow
@@ -414,7 +415,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
                                                   T* __restrict__ xmax_acc, const K1Item* __restrict__ items,
                                                   int nitems, const double* __restrict__ part,
                                                   const int* __restrict__ part_range, int ndt,
-                                                  unsigned* __restrict__ done) {
+                                                  const int* __restrict__ dt_list, unsigned* __restrict__ done) {
   constexpr int NW = NT / 64;
   constexpr int QN = 16 / sizeof(T);
   constexpr int QC = (13 * CPL + QN - 1) / QN;  // 16-byte words per lane: cells
@@ -628,7 +629,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
     }
     __syncthreads();
     if (last) {
-      for (int s = 0; s < ndt; ++s) dt_control<T, NT>(s, sims, st, xmax_acc, part, part_range, 1, DRED, tid);
+      for (int s = 0; s < ndt; ++s)
+        dt_control<T, NT>(dt_list ? dt_list[s] : s, sims, st, xmax_acc, part, part_range, 1, DRED, tid);
       if (tid == 0) atomicExch(done, 0u);
     }
   }

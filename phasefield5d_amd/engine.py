@@ -172,6 +172,12 @@ class Engine:
         _lib.check(_lib.lib().pf5d_engine_time_kernel(self.h, w, int(reps), C.byref(ms)))
         return ms.value
 
+    def num_groups(self):
+        """Simulation groups stepped one after another (1: all step together)."""
+        n = C.c_int()
+        _lib.check(_lib.lib().pf5d_engine_num_groups(self.h, C.byref(n)))
+        return n.value
+
     def launch_info(self):
         out = (C.c_long * 12)()
         _lib.check(_lib.lib().pf5d_engine_launch_info(self.h, out))
