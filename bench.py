@@ -12,6 +12,8 @@ Reported:
   fp64_plain   the per-function drop-in FP64 path (one device call per reference
                function, host dt from abs_max each step)
   error        max / RMS |X_fp32 - X_fp64| after the timed steps (same init)
+  fp32_batched_128  eight 128^3 simulations in one FP32 engine (stepped in
+               groups that fit the last-level cache), and its ratio to fp64_plain
   fp32_developed_field  the FP32 step on a decomposed-looking field (smooth
                modes of amplitude 0.05 per component), as later in a run
   roofline     the dominant kernel of the FP32 step: with the two-pass stencil
@@ -211,6 +213,28 @@ def main():
         ed.close()
         log("fp32 developed field", developed)
 
+    # batched FP32 at 128^3: eight simulations of the workload in one engine (the
+    # engine steps them in groups that fit the last-level cache)
+    batched128 = None
+    if not args.fp32_only:
+        nb = 8
+        eb = Engine("fp32")
+        for _ in range(nb):
+            eb.add_problem(pb)
+        eb.set_tiling(info["variant"], info["cx"])
+        eb.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
+        eb.step(8)
+        eb.sync()
+        kb = 48
+        t0 = time.perf_counter()
+        eb.step(kb)
+        eb.sync()
+        elb = time.perf_counter() - t0
+        batched128 = {"sims": nb, "steps": kb, "groups": eb.num_groups(), "cell_updates_per_s": nb * ncell * kb / elb,
+                      "ms_per_step": 1e3 * elb / kb}
+        eb.close()
+        log("fp32 batched 8 x 128^3", batched128)
+
     mass0 = pb.X_init.reshape(-1, 4).sum(axis=0)
     drift32 = np.abs(st32["mass"] - mass0) / mass0
     v64 = k1_64 = el64 = vplain = None
@@ -283,6 +307,9 @@ def main():
         "fp64_fused": {"value": v64, "ms_per_step": 1e3 * el64 / args.steps if el64 else None, "k1_ms": k1_64},
         "fp64_plain": {"value": vplain, "steps": args.plain_steps},
         "ratio_fp32_vs_plain_fp64": v32 / vplain if vplain else None,
+        "fp32_batched_128": batched128,
+        "ratio_batched_128_vs_plain_fp64": (batched128["cell_updates_per_s"] / vplain
+                                            if batched128 and vplain else None),
         "error_vs_fp64": {"max": float(diff.max()), "rms": float(np.sqrt((diff ** 2).mean())),
                           "steps": args.warmup + args.steps},
         "mass_drift_fp32": drift32.tolist(),
