@@ -844,9 +844,10 @@ struct Engine : EngineBase {
   bool dirty = true;
   hipGraphExec_t g1 = nullptr, gN = nullptr;
   static constexpr int GN = 8;
+  static constexpr int GNG = 16;  // steps per graph of a group (its fields stay cached meanwhile)
   unsigned k1_grid = 0, kf_grid = 0;
   // Simulations whose per-step working sets together exceed what the 256 MB
-  // Infinity Cache holds are stepped in groups: each group runs GN steps (its
+  // Infinity Cache holds are stepped in groups: each group runs GNG steps (its
   // own graph) while its fields stay in the cache, then the next group.  Every
   // simulation advances by the same steps; only the order of independent work
   // changes.  A group is a range of the work lists, its own FFT batch and the
@@ -1431,11 +1432,11 @@ struct Engine : EngineBase {
       for (auto& gp : groups) {
         Group& g = *gp;
         if (!g.g1) g.g1 = capture_with(1, [&] { enqueue_group_step(g); });
-        if (n >= GN && !g.gN) g.gN = capture_with(GN, [&] { enqueue_group_step(g); });
+        if (n >= GNG && !g.gN) g.gN = capture_with(GNG, [&] { enqueue_group_step(g); });
       }
-      while (n >= GN) {
+      while (n >= GNG) {
         for (auto& g : groups) HIPCHK(hipGraphLaunch(g->gN, stream));
-        n -= GN;
+        n -= GNG;
       }
       for (auto& g : groups)
         for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(g->g1, stream));

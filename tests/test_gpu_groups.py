@@ -41,7 +41,7 @@ def test_groups_bitwise_equal_alone(dev, precision):
         p = problem(2, 32, seed=40 + k)
         p.X_init = developed_state(p, amp=0.03, seed=k)
         pbs.append(p)
-    steps = 11  # one graph of 8 steps per group, then 3 single steps
+    steps = 19  # one graph of 16 steps per group, then 3 single steps
     e = Engine(precision)
     slots = [e.add_problem(p) for p in pbs]
     assert e.num_groups() == 3
