@@ -860,6 +860,9 @@ struct Engine : EngineBase {
     int* d_list = nullptr;
     FftBatch<T> fft;
     hipGraphExec_t g1 = nullptr, gN = nullptr;
+    Group() = default;
+    Group(const Group&) = delete;  // owns device memory and graphs
+    Group& operator=(const Group&) = delete;
     void drop() {
       if (g1) (void)hipGraphExecDestroy(g1);
       if (gN) (void)hipGraphExecDestroy(gN);
