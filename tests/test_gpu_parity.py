@@ -95,11 +95,12 @@ def test_scalar_kernels(dev):
 
 # lengths with large prime factors (101, 193; 7 x 11 x 13) run the direct-DFT passes;
 # FP64 lines of 3200 (the longest 1D request) and 4002 = 2 x 3 x 23 x 29 keep their
-# twiddles in global memory
+# twiddles in global memory; x lengths 64 and 128 and a y length of 64 take the
+# compile-time plans along strided axes (lines gathered into LDS)
 @pytest.mark.parametrize("dim,shape", [(1, (160,)), (1, (189,)), (1, (808,)), (1, (2316,)), (1, (1001,)), (1, (3200,)),
                                        (1, (4002,)), (2, (48, 96)),
                                        (2, (64, 64)), (2, (52, 22)), (3, (16, 24, 32)), (3, (12, 12, 12)),
-                                       (3, (14, 22, 26))])
+                                       (3, (14, 22, 26)), (3, (64, 12, 10)), (3, (12, 64, 10)), (2, (128, 20))])
 def test_elastic_update(dev, dim, shape):
     from phasefield5d_amd.solver.elastic import make_elastic_updater
     from phasefield5d_amd.solver.system import (build_fourier_grid_1d_along_direction,
