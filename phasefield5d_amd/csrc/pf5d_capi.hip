@@ -892,14 +892,18 @@ struct Engine : EngineBase {
       if (!v.split())
         (void)hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds);
     // default tiling (until set_tiling): the marching two-pass form with 16 x 16
-    // potentials tiles and the flux pass at three waves per SIMD, the
-    // autotuner's pick for 3D fields from 64^3 to 256^3 in both precisions
-    // (about twice as fast there as the single-pass tilings); x chunks of 16
-    // planes; FFT lines of about six points per thread (32 KiB, 256 threads)
-    // and two (FP32) or four (FP64) parts per plane of the 128 x 128 plane passes
+    // potentials tiles, the flux pass with two cells per lane at two waves per
+    // SIMD in FP32 and one cell at three in FP64, the autotuner's pick for 3D
+    // fields from 64^3 to 256^3 (about twice as fast there as the single-pass
+    // tilings); x chunks of 16 planes; FFT lines of about six points per thread
+    // (32 KiB, 256 threads) and two (FP32) or four (FP64) parts per plane of the
+    // 128 x 128 plane passes
+    const int want_cpl = sizeof(T) == 4 ? 2 : 1, want_minw = sizeof(T) == 4 ? 1 : 3;
     variant = -1;
     for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
-      if (variants[i].fcpl > 0 && variants[i].fminw == 3 && variants[i].bz == 16) variant = i;
+      if (variants[i].fcpl == want_cpl && variants[i].fminw == want_minw && variants[i].fnt == 256 &&
+          variants[i].bz == 16)
+        variant = i;
     for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
       if (!variants[i].split() && variants[i].lds <= (size_t)devinfo().lds) variant = i;
     require(variant >= 0, "no stencil tiling fits this device's LDS");

@@ -481,7 +481,19 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
 #pragma unroll
       for (int j = 0; j < CPL; ++j) face_ab<T>(pv[j], me[j], inv_dx, fxd[j]);
     }
+    // FP32: the tile's edge wavefronts load the halo row beyond it at the start
+    // of each plane, so the load overlaps the exchange instead of stalling the
+    // block after its first barrier (a tile of one row loads both in place).
+    // FP64 loads them in place: the extra row of registers made it spill
+    constexpr bool prefetch_halo = sizeof(T) == 4;
+    const bool pre_up = prefetch_halo && hy && !ywrap && item.ty > 1 && w == item.ty - 1;
+    const bool pre_dn = prefetch_halo && hy && !ywrap && item.ty > 1 && w == 0;
     for (int x = item.xa; x < item.xb; ++x) {
+      CellV<T> hal[CPL];
+      if (pre_up)
+        load_lane(x, gy + 1 < n1 ? gy + 1 : 0, hal);
+      else if (pre_dn)
+        load_lane(x, gy > 0 ? gy - 1 : n1 - 1, hal);
       CellV<T> nx[CPL];
       if (hx) load_lane(x + 1 < n0 ? x + 1 : 0, gy, nx);
       T fyu[CPL][4], fyd[CPL][4];
@@ -494,7 +506,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
             get_cells<T, CPL, QC>(SC[w + 1], lane, up);
           else if (ywrap)
             get_cells<T, CPL, QC>(SC[0], lane, up);
-          else
+          else if (pre_up) {
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) up[j] = hal[j];
+          } else
             load_lane(x, gy + 1 < n1 ? gy + 1 : 0, up);
 #pragma unroll
           for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], up[j], inv_dx, fyu[j]);
@@ -531,7 +546,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
             for (int a = 0; a < 4; ++a) fyd[j][a] = p[4 * j + a];
         } else {
           CellV<T> dn[CPL];
-          load_lane(x, gy > 0 ? gy - 1 : n1 - 1, dn);
+          if (pre_dn) {
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) dn[j] = hal[j];
+          } else {
+            load_lane(x, gy > 0 ? gy - 1 : n1 - 1, dn);
+          }
 #pragma unroll
           for (int j = 0; j < CPL; ++j) face_ab<T>(dn[j], me[j], inv_dx, fyd[j]);
         }
