@@ -2,7 +2,7 @@
 
 Bars: per-function FP64 kernels without FFT are bit-exact (same operation
 order, no contraction); anything involving the FFT pair is held to 1e-12
-normwise relative (north-star FP64 tolerance); FP32 results to 1e-5 absolute in
+normwise relative (the FP64 per-step tolerance); FP32 results to 1e-5 absolute in
 composition after N steps.
 """
 import numpy as np
