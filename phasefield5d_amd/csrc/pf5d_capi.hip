@@ -1128,6 +1128,7 @@ struct Engine : EngineBase {
     }
     v.elastic = d.elastic;
     v.use_mean = (sizeof(T) == 4) ? 1 : 0;
+    v.fresh = s.fresh ? 1 : 0;
     v.tab = view<T>(s.tab);
     v.X = s.X;
     v.Xlo = s.Xlo;
@@ -1322,12 +1323,23 @@ struct Engine : EngineBase {
       occ = std::max(1, occ);
       k1_grid = (unsigned)std::max(1, std::min(nk1, devinfo().cus * occ));
     }
-    // prime after every rebuild (the partial-sum buffer is new): misfit and FP64
-    // statistics of the current X; X is not modified, so a rebuild between two
-    // steps leaves every simulation's trajectory unchanged
+    // prime after every rebuild (the partial-sum buffer is new): FP64 statistics
+    // of the current X, and the misfit of fields just set; X and the other
+    // misfits are not modified, so a rebuild between two steps leaves every
+    // simulation's trajectory unchanged
     if (nk2) {
       launch_k2(1);
       launch_dt(0);
+    }
+    // later prime launches (state reads) keep the misfit the steps write
+    bool any_fresh = false;
+    for (auto& v : devs) {
+      any_fresh = any_fresh || v.fresh;
+      v.fresh = 0;
+    }
+    if (any_fresh) {
+      HIPCHK(hipMemcpyAsync(d_sims, devs.data(), cap * sizeof(SimDev<T>), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
     }
     for (auto& s : sims) s.fresh = false;
     dirty = false;

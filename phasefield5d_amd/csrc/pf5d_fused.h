@@ -36,6 +36,7 @@ struct SimDev {
   T X0[4];            // misfit origin: X0 (FP64 mode), FP64 mean of the initial field (FP32 mode)
   int elastic;
   int use_mean;       // 1 when X0 holds the initial mean (the k=0 mode is removed before the FP32 FFT)
+  int fresh;          // the field was just set: a prime launch writes its misfit s
   TableView<T> tab;
   T* X;
   T* Xlo;             // FP32 mode: rounding remainder, the state is X + Xlo (null in FP64 mode)
@@ -581,7 +582,11 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
           st4g<T>(sd.Xlo + 4 * c, lo);
         }
       }
-      if (sd.elastic && run) {
+      // the misfit s: from the updated state in a step; a prime launch writes it
+      // only for a field just set, and otherwise keeps the one the last step
+      // wrote (X + X_lo recombined can differ from the updated FP64 value
+      // below FP32 resolution, so recomputing it could round differently)
+      if (sd.elastic && run && (!prime || sd.fresh)) {
         T acc = (T)(xd[0] - (double)ref[0]) * sd.l[0];
 #pragma unroll
         for (int a = 1; a < 4; ++a) acc = mad<T>((T)(xd[a] - (double)ref[a]), sd.l[a], acc);

@@ -55,3 +55,33 @@ def test_groups_bitwise_equal_alone(dev, precision):
         assert np.array_equal(Xa, Xb), f"simulation {i} {p.shape}"
         assert sta["dt"] == stb["dt"] and sta["t"] == stb["t"]
         assert np.array_equal(sta["mass"], stb["mass"])
+
+
+def test_groups_rebuilt_after_removal(dev):
+    """Removing simulations between steps regroups the rest; each survivor's
+    trajectory equals the same simulation stepped alone."""
+    from phasefield5d_amd.engine import Engine
+    from phasefield5d_amd.problem import build_problem
+    from phasefield5d_amd.workloads import workload_config
+    pbs = [build_problem(workload_config("3d_128", seed=s)) for s in (0, 1, 2)]
+    for k in range(3):
+        p = problem(2, 32, seed=60 + k)
+        p.X_init = developed_state(p, amp=0.03, seed=k)
+        pbs.append(p)
+    e = Engine("fp32")
+    slots = [e.add_problem(p) for p in pbs]
+    assert e.num_groups() == 4  # one per 128^3 field, one for the three small ones
+    e.step(17)
+    e.remove(slots[1])  # a 128^3 group disappears
+    e.remove(slots[4])  # a small one from the last group
+    assert e.num_groups() == 3
+    e.step(6)
+    keep = [0, 2, 3, 5]
+    got = {i: (e.field(slots[i], "X"), e.state(slots[i])) for i in keep}
+    e.close()
+    for i in keep:
+        Xa, sta = _alone(pbs[i], "fp32", 23)
+        Xb, stb = got[i]
+        assert stb["steps"] == 23
+        assert np.array_equal(Xa, Xb), f"simulation {i} {pbs[i].shape}"
+        assert sta["dt"] == stb["dt"] and sta["t"] == stb["t"]
