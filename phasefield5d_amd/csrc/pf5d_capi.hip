@@ -371,10 +371,16 @@ struct FftBatch {
     const auto it = plans.index.find(n);
     return (it != plans.index.end() && plans.plans[it->second].tw_global) ? 0 : (size_t)fft_head_slots(n);
   }
+  // LDS bytes per complex line of an item: two line buffers, and the Bhat
+  // values CONV items of the compile-time lengths gather next to them
+  static size_t line_bytes(int n, int kind) {
+    return 2 * (size_t)line_stride(n) * sizeof(cx<T>) +
+           ((fft_ct_length(n) && kind == FK_CONV) ? (size_t)n * sizeof(T) : 0);
+  }
   int pick_g(int n, long nlines, int kind) const {
     const size_t head = head_slots(n) * sizeof(cx<T>);
     const size_t avail = budget > head ? budget - head : 0;
-    long g = (long)(avail / (2 * (size_t)line_stride(n) * sizeof(cx<T>)));
+    long g = (long)(avail / line_bytes(n, kind));
     g = std::max<long>(1, std::min<long>(g, 128));
     if (paired(kind)) g *= 2;
     return (int)std::min<long>(g, nlines);
@@ -382,7 +388,7 @@ struct FftBatch {
   void push(int st, FftItem it, int n) {
     const size_t gl = paired(it.kind) ? (it.G + 1) / 2 : it.G;
     stage[st].push_back(it);
-    lds[st] = std::max(lds[st], (head_slots(n) + 2 * gl * line_stride(n)) * sizeof(cx<T>));
+    lds[st] = std::max(lds[st], head_slots(n) * sizeof(cx<T>) + gl * line_bytes(n, it.kind));
   }
   // lines [0, nlines) with line l at base + l*ls; split into groups of G (a
   // multiple of align lines when G >= align)

@@ -484,6 +484,10 @@ struct CtRun {
 // dst[slot(idx)] = fetch(idx) for idx = tid, tid + NT, ... < tot, FFT_LB at a
 // time: all fetches of a batch are issued before the first LDS write
 constexpr int FFT_LB = 4;
+// lengths with a compile-time plan (the cases of k_fft's dispatch)
+PF5D_HD constexpr bool fft_ct_length(int n) {
+  return n == 64 || n == 96 || n == 128 || n == 192 || n == 256 || n == 512;
+}
 template <typename T, int NT, typename F>
 PF5D_DEV void gather_lds(int tot, int tid, cx<T>* dst, F fetch) {
 #pragma unroll 1
@@ -501,7 +505,8 @@ PF5D_DEV void gather_lds(int tot, int tid, cx<T>* dst, F fetch) {
   }
 }
 
-// LDS: the plan's twiddles (N slots), then two buffers of GL lines
+// LDS: the plan's twiddles (N slots), then two buffers of GL lines (then, for
+// CONV items, their GL * N Bhat values)
 template <typename T, int NT, int N>
 PF5D_DEV void fft_item_ct(const FftItem& I, const cx<T>* twl, cx<T>* b0, int tid) {
   constexpr int LS = line_stride(N), NH = N / 2 + 1;
@@ -518,7 +523,38 @@ PF5D_DEV void fft_item_ct(const FftItem& I, const cx<T>* twl, cx<T>* b0, int tid
   // loads coalesce) into b1 by every thread, then every pass runs from LDS.
   // (Reading them in the first pass instead, with only its butterflies' threads
   // loading, was slower: the x pass at 128^3 took 4 us more in FP32, 9 in FP64.)
-  if (fuse_in) {
+  // CONV items gather their Bhat with the lines, into GL * N values after the
+  // two line buffers (slot m GL + g), for the first transform's last pass
+  const bool conv_bl = fuse_in && I.kind == FK_CONV;
+  T* bl = reinterpret_cast<T*>(b1 + GL * LS);
+  if (conv_bl) {
+    const int tot = GL * N;
+#pragma unroll 1
+    for (int base = tid; base < tot; base += NT * FFT_LB) {
+      cx<T> v[FFT_LB];
+      T bb[FFT_LB];
+      int at[FFT_LB];
+#pragma unroll
+      for (int u = 0; u < FFT_LB; ++u) {
+        const int idx = base + u * NT;
+        at[u] = -1;
+        if (idx < tot) {
+          int m, g;
+          fdivmod(idx, GL, rgl, m, g);
+          at[u] = g * LS + lpad(m);
+          v[u] = item_input<T>(I, G, N, NH, g, m);
+          bb[u] = item_bhat<T>(I, N, NH, g, m);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < FFT_LB; ++u)
+        if (at[u] >= 0) {
+          b1[at[u]] = v[u];
+          bl[base + u * NT] = bb[u];
+        }
+    }
+    __syncthreads();
+  } else if (fuse_in) {
     gather_lds<T, NT>(GL * N, tid, b1, [&](int idx, int& dst) -> cx<T> {
       int m, g;
       fdivmod(idx, GL, rgl, m, g);
@@ -549,7 +585,7 @@ PF5D_DEV void fft_item_ct(const FftItem& I, const cx<T>* twl, cx<T>* b0, int tid
   if (I.kind == FK_CONV || I.kind == FK_LINE) {
     // forward; W = conj(Z) * Bhat in natural order; forward again (from that buffer)
     auto sink_bhat = [&](int g, int m, const cx<T>& z, cx<T>* other) {
-      const T b = item_bhat<T>(I, N, NH, g, m);
+      const T b = conv_bl ? bl[m * GL + g] : item_bhat<T>(I, N, NH, g, m);
       other[g * LS + lpad(m)] = {z.x * b, -z.y * b};
     };
     CtRun<T, NT, N, 0, 1>::run(GL, rgl, tid, twl, b0, b1, false, fetch_b1, false, sink_bhat);
@@ -689,7 +725,7 @@ __global__ void __launch_bounds__(NT) k_fft(const FftItem* __restrict__ items, i
       __syncthreads();
     }
     cx<T>* buf = tg ? twl : twl + pl.n + pl.nroot;
-    switch (pl.n) {
+    switch (pl.n) {  // the lengths of fft_ct_length
       case 64: fft_item_ct<T, NT, 64>(I, twl, buf, tid); break;
       case 96: fft_item_ct<T, NT, 96>(I, twl, buf, tid); break;
       case 128: fft_item_ct<T, NT, 128>(I, twl, buf, tid); break;
