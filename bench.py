@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--error-steps", type=int, default=5000,
+                    help="steps of the untimed FP32-vs-FP64 precision run (0: only the timed steps)")
     ap.add_argument("--variant", type=int, default=None)
     ap.add_argument("--cx", type=int, default=None)
     ap.add_argument("--calibration-copy", action="store_true",
@@ -187,6 +189,22 @@ def main():
         kf_ms = e32.time_kernel("k1_flux", reps=20)
     log(f"fp32: {v32:.4g} cell-updates/s, {1e3 * el32 / args.steps:.3f} ms/step; K1 {k1_ms:.3f} ms, FFT {fft_ms:.3f} ms",
         info)
+    # per-stage durations inside whole steps (HIP events between the stages of steps
+    # launched one by one), on a second engine holding the same workload, so the
+    # headline simulation keeps its step count
+    march = split and e32.variants()[info["variant"]][3] > 0
+    in_step = in_step_sep = None
+    if march:
+        et = Engine("fp32")
+        et.add_problem(pb)
+        et.set_tiling(info["variant"], info["cx"])
+        et.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
+        et.step(args.warmup)
+        et.sync()
+        in_step = et.time_step_stages(max(args.steps, 50))
+        in_step_sep = et.time_step_stages(50, separate_dt=True)
+        et.close()
+        log("fp32 stages in the step (ms)", in_step, "dt as its own launch", in_step_sep)
     st32 = e32.state(s32)
     if not (st32["status"] & 1) or st32["steps"] != args.warmup + args.steps:
         raise RuntimeError(f"fp32 simulation stopped early: {st32}")
@@ -239,6 +257,7 @@ def main():
     drift32 = np.abs(st32["mass"] - mass0) / mass0
     v64 = k1_64 = el64 = vplain = None
     diff = np.zeros(1)
+    long_err = None
     if not args.fp32_only:
         # FP64 fused, same number of steps from the same state (error reference)
         log("fp64 fused")
@@ -248,6 +267,29 @@ def main():
         X32, X64 = e32.field(s32, "X"), e64.field(s64, "X")
         diff = np.abs(X32 - X64)
         log(f"fp64 fused: {v64:.4g} cell-updates/s; max|dX| {diff.max():.3e} rms {np.sqrt((diff**2).mean()):.3e}")
+        # the precision run: both engines continue, untimed, to the run length of the
+        # 3D 128^3 precision workload (5000 steps); composition error and mass drift there
+        done = args.warmup + args.steps
+        if args.error_steps > done:
+            t0 = time.perf_counter()
+            for eng in (e32, e64):
+                left = args.error_steps - done
+                while left > 0:
+                    eng.step(min(left, 500))
+                    left -= 500
+                eng.sync()
+            st32e, st64e = e32.state(s32), e64.state(s64)
+            X32e, X64e = e32.field(s32, "X"), e64.field(s64, "X")
+            de = np.abs(X32e - X64e)
+            long_err = {"steps": args.error_steps, "steps_fp32": st32e["steps"], "steps_fp64": st64e["steps"],
+                        "max": float(de.max()), "rms": float(np.sqrt((de ** 2).mean())),
+                        "t_fp32": st32e["t"], "t_fp64": st64e["t"],
+                        "active": bool(st32e["status"] & 1 and st64e["status"] & 1),
+                        "mass_drift_fp32": (np.abs(st32e["mass"] - mass0) / mass0).tolist(),
+                        "mass_drift_fp64": (np.abs(st64e["mass"] - mass0) / mass0).tolist(),
+                        "field_mass_drift_fp32": (np.abs(X32e.reshape(-1, 4).sum(axis=0) - mass0) / mass0).tolist(),
+                        "seconds": time.perf_counter() - t0}
+            log("fp32 vs fp64 over the precision run", long_err)
         del e64
 
         # plain FP64 drop-in path
@@ -271,25 +313,32 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    def roofline(kname, kfile, bpc, kms):
+    def roofline(kname, kfile, bpc, kms, alone_ms=None):
+        """achieved = algorithmic bytes / the launch's average duration inside the
+        step (kms); alone_ms: the same kernel launched repeatedly on its own"""
         traffic = None
         tpath = os.path.join(ROOT, "profiles", kfile) if kfile else None
         if tpath and os.path.exists(tpath):
             with open(tpath) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
         achieved = bpc * ncell / (kms * 1e-3) / 1e9
-        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname, "bytes_per_cell": bpc,
-                "avg_ms": kms}
+        r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname, "bytes_per_cell": bpc,
+             "avg_ms": kms}
+        if alone_ms is not None:
+            r["alone_ms"] = alone_ms
+            r["frac_alone"] = bpc * ncell / (alone_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+        return r
 
     roof_other = None
     if split:
-        march = e32.variants()[info["variant"]][3] > 0
+        kp_in = in_step["k1_potentials"] if in_step else kp_ms
+        kf_in = in_step["k1_flux"] if in_step else kf_ms
         passes = [roofline("k_potentials<float> (interpolation from the LDS node box + kappa lap + psi -> mu, Mob)",
-                           "k_potentials_3d128_fp32_traffic.json", 56, kp_ms),
+                           "k_potentials_3d128_fp32_traffic.json", 56, kp_in, kp_ms),
                   roofline("k_fluxmarch<float> (face fluxes + divergence from X, mu, Mob; dt controller in its "
                            "last block)" if march else "k_fluxdiv<float> (face fluxes + divergence)",
-                           "k_fluxmarch_3d128_fp32_traffic.json" if march else None, 68, kf_ms)]
+                           "k_fluxmarch_3d128_fp32_traffic.json" if march else None, 68, kf_in, kf_ms)]
         passes.sort(key=lambda r: -r["avg_ms"])
         roof, roof_other = passes
     else:
@@ -313,9 +362,12 @@ def main():
         "error_vs_fp64": {"max": float(diff.max()), "rms": float(np.sqrt((diff ** 2).mean())),
                           "steps": args.warmup + args.steps},
         "mass_drift_fp32": drift32.tolist(),
+        "error_vs_fp64_long": long_err,
         "fp32_developed_field": developed,
         "stage_ms": {"k1": k1_ms, "k1_potentials": kp_ms, "k1_flux": kf_ms, "fft": fft_ms, "k2": k2_ms,
                      "dt": dt_ms},
+        "stage_ms_in_step": in_step,
+        "stage_ms_in_step_dt_launch": in_step_sep,
         "launch": info,
     }
     if not args.fp32_only and not args.no_mix:

@@ -193,6 +193,15 @@ int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups);
  * state of the step unchanged. */
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms);
 
+/* per-stage device time of whole steps of a one-group engine with the marching two-pass
+ * stencil: reps steps launched one by one on the engine stream with HIP events between
+ * the stages.  out[6] = average ms of the FFT passes, the potentials pass, the flux pass
+ * (its last block running the dt controller unless separate_dt), the separate dt launch
+ * (0 when fused), the update pass and the whole step.  The steps advance the simulations
+ * as pf5d_engine_step(e, reps) does.  A measurement helper with no counterpart in the
+ * reference (solver/system.py steps without per-stage timing). */
+int pf5d_engine_time_step_stages(pf5d_engine* e, int reps, int separate_dt, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,7 @@ _SIGS = {
     "pf5d_engine_launch_info": (i32, [vp, P_i64]),
     "pf5d_engine_num_groups": (i32, [vp, P_i32]),
     "pf5d_engine_time_kernel": (i32, [vp, i32, i32, P_f64]),
+    "pf5d_engine_time_step_stages": (i32, [vp, i32, i32, P_f64]),
 }
 
 EXPORTED = tuple(_SIGS)

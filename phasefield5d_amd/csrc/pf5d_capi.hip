@@ -738,6 +738,7 @@ struct EngineBase {
   virtual void launch_info(long* out) = 0;
   virtual int num_groups() = 0;
   virtual double time_kernel(int which, int reps) = 0;
+  virtual void time_step_stages(int reps, int separate_dt, double* out) = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
 };
@@ -1624,6 +1625,67 @@ struct Engine : EngineBase {
     (void)hipEventDestroy(b);
     return (double)ms / reps;
   }
+  // per-stage device time of whole steps: reps steps launched one by one (not
+  // from the step graph) on the engine stream, with HIP events between the
+  // stages; out[0..5] = average ms of the FFT passes, potentials pass, flux
+  // pass (with the dt controller in its last block unless separate_dt),
+  // separate dt launch (0 when fused), update pass, whole step.  The steps
+  // advance the simulations exactly as step(reps) does.
+  void time_step_stages(int reps, int separate_dt, double* out) override {
+    require(reps > 0 && reps <= 4096, "time_step_stages: reps must be in 1..4096");
+    if (dirty) rebuild();
+    require(groups.size() <= 1, "time_step_stages: the simulations form more than one group");
+    const K1Variant<T>& V = variants[variant];
+    require(V.split() && V.fcpl > 0, "time_step_stages: the marching two-pass stencil is not selected");
+    require(nk1 > 0 && nk2 > 0, "time_step_stages: no active simulation");
+    constexpr int NE = 6;
+    std::vector<hipEvent_t> ev((size_t)NE * reps, nullptr);
+    auto destroy = [&] {
+      for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    };
+    try {
+      for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+      bool fused = false;
+      for (int r = 0; r < reps; ++r) {
+        hipEvent_t* e = &ev[(size_t)NE * r];
+        HIPCHK(hipEventRecord(e[0], stream));
+        fft.run(stream, d_state);
+        HIPCHK(hipEventRecord(e[1], stream));
+        hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e[2], stream));
+        fused = launch_flux(V, !separate_dt);
+        HIPCHK(hipGetLastError());
+        if (!fused) {
+          HIPCHK(hipEventRecord(e[3], stream));
+          launch_dt(1);
+        }
+        HIPCHK(hipEventRecord(e[4], stream));
+        launch_k2(0);
+        HIPCHK(hipEventRecord(e[5], stream));
+      }
+      stats_stale = true;
+      HIPCHK(hipEventSynchronize(ev.back()));
+      double acc[NE] = {0, 0, 0, 0, 0, 0};
+      for (int r = 0; r < reps; ++r) {
+        hipEvent_t* e = &ev[(size_t)NE * r];
+        float ms = 0;
+        for (int i = 0; i + 1 < NE; ++i) {
+          if (fused && i == 3) continue;  // no separate dt launch: the flux stage ends at e[4]
+          HIPCHK(hipEventElapsedTime(&ms, e[i], e[(fused && i == 2) ? 4 : i + 1]));
+          acc[i] += ms;
+        }
+        HIPCHK(hipEventElapsedTime(&ms, e[0], e[NE - 1]));
+        acc[NE - 1] += ms;
+      }
+      for (int i = 0; i < NE; ++i) out[i] = acc[i] / reps;
+    } catch (...) {
+      destroy();
+      throw;
+    }
+    destroy();
+  }
   void launch_info(long* out) override {
     if (dirty) rebuild();
     out[0] = nk1;
@@ -2076,6 +2138,9 @@ int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes) {
 }
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms) {
   return guarded([&] { *avg_ms = e->impl->time_kernel(which, reps); });
+}
+int pf5d_engine_time_step_stages(pf5d_engine* e, int reps, int separate_dt, double* out) {
+  return guarded([&] { e->impl->time_step_stages(reps, separate_dt, out); });
 }
 int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups) {
   return guarded([&] { *ngroups = e->impl->num_groups(); });

@@ -403,10 +403,18 @@ PF5D_DEV void dt_control(int sim, const SimDev<T>* __restrict__ sims, SimState* 
                          T* __restrict__ xmax_acc, const double* __restrict__ part,
                          const int* __restrict__ part_range, int advance, double (*red)[NSTAT], int tid) {
   const int p0 = part_range[2 * sim], p1 = part_range[2 * sim + 1];
+  // The update pass writes the partials only in a prime launch and ahead of a
+  // bounds check (when its step count is a multiple of the check interval); a
+  // stepping controller sees that same count, so on other steps the partials
+  // are the ones already reduced into the state, and are not read again
+  // (uniform over the block: every thread reads the same state)
+  const int iv = sims[sim].oob_interval;
+  const bool reduce = !advance || (iv > 0 && (st[sim].steps % iv) == 0);
+  const int q1 = reduce ? p1 : p0;
   // Neumaier sums of the mass partials, min/max of the rest
   double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
   double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
-  for (int i = p0 + tid; i < p1; i += NTB) {
+  for (int i = p0 + tid; i < q1; i += NTB) {
     const double* q = part + (long)i * NSTAT;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -449,7 +457,7 @@ PF5D_DEV void dt_control(int sim, const SimDev<T>* __restrict__ sims, SimState* 
       // statistics of the fields as the last update (or a prime launch) left
       // them: kept for inactive simulations only from a prime launch (advance
       // 0), which computes them from the current fields
-      if (p1 > p0 && (!advance || (S.status & ST_ACTIVE))) {
+      if (reduce && p1 > p0 && (!advance || (S.status & ST_ACTIVE))) {
         for (int a = 0; a < 4; ++a) {
           S.mass[a] = v[a];
           S.mean[a] = v[a] / (double)sd.ncell;

@@ -172,6 +172,14 @@ class Engine:
         _lib.check(_lib.lib().pf5d_engine_time_kernel(self.h, w, int(reps), C.byref(ms)))
         return ms.value
 
+    def time_step_stages(self, reps=50, separate_dt=False):
+        """Average ms per stage over ``reps`` whole steps launched one by one with HIP
+        events between the stages (one-group engine, marching two-pass stencil).  The
+        steps advance the simulations as ``step(reps)`` does."""
+        out = (C.c_double * 6)()
+        _lib.check(_lib.lib().pf5d_engine_time_step_stages(self.h, int(reps), 1 if separate_dt else 0, out))
+        return dict(zip(("fft", "k1_potentials", "k1_flux", "dt", "k2", "step"), list(out)))
+
     def num_groups(self):
         """Simulation groups stepped one after another (1: all step together)."""
         n = C.c_int()

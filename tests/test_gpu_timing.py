@@ -1,0 +1,55 @@
+"""Per-stage timing of whole steps: the timed steps advance a simulation exactly as
+the captured step graph does, and every stage reports a duration."""
+import numpy as np
+import pytest
+
+from helpers import problem
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from phasefield5d_amd import _lib
+    _lib.require_gpu()
+    return _lib
+
+
+def _run(pb, precision, advance):
+    from phasefield5d_amd.engine import Engine
+    e = Engine(precision)
+    s = e.add_problem(pb)
+    out = advance(e)
+    X, st = e.field(s, "X"), e.state(s)
+    e.close()
+    return X, st, out
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("separate_dt", [False, True])
+def test_stage_timing_steps_like_graph(dev, precision, separate_dt):
+    pb = problem(3, 24, seed=7)
+    n = 9
+    X_ref, st_ref, _ = _run(pb, precision, lambda e: e.step(n))
+    X, st, t = _run(pb, precision, lambda e: e.time_step_stages(n, separate_dt))
+    assert st["steps"] == st_ref["steps"] == n
+    assert st["dt"] == st_ref["dt"] and st["t"] == st_ref["t"]
+    assert np.array_equal(X, X_ref)
+    for k in ("fft", "k1_potentials", "k1_flux", "k2", "step"):
+        assert t[k] > 0, (k, t)
+    assert (t["dt"] > 0) == separate_dt, t
+    parts = t["fft"] + t["k1_potentials"] + t["k1_flux"] + t["dt"] + t["k2"]
+    assert abs(parts - t["step"]) <= 1e-3 + 1e-3 * t["step"], t
+
+
+def test_stage_timing_rejects_groups(dev):
+    from phasefield5d_amd.engine import Engine
+    from phasefield5d_amd.problem import build_problem
+    from phasefield5d_amd.workloads import workload_config
+    e = Engine("fp32")
+    for s in (0, 1):
+        e.add_problem(build_problem(workload_config("3d_128", seed=s)))
+    assert e.num_groups() == 2
+    with pytest.raises(ValueError, match="more than one group"):
+        e.time_step_stages(2)
+    e.close()
