@@ -201,7 +201,7 @@ def main():
         et.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
         et.step(args.warmup)
         et.sync()
-        in_step = et.time_step_stages(max(args.steps, 50))
+        in_step = et.time_step_stages(min(max(args.steps, 50), 4096))
         in_step_sep = et.time_step_stages(50, separate_dt=True)
         et.close()
         log("fp32 stages in the step (ms)", in_step, "dt as its own launch", in_step_sep)
