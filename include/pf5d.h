@@ -103,6 +103,16 @@ int pf5d_clamp_simplex(int prec, void* X, long ncell, void* stream);
  * [nonfinite count, max sum, min sum, min component, max component]; blocks. */
 int pf5d_bounds_stats(int prec, const void* X, long ncell, double host_out5[5], void* stream);
 
+/* pf5d_bounds_stats without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernel, same result bits.
+ * The per-block partials live in a fixed 40 KiB device array (1024 blocks of five
+ * doubles; the launch uses at most two blocks per CU) that each GPU holds until
+ * the process ends; a device needing more blocks is rejected with PF5D_EINVAL.
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * (other host threads, other streams) race; serialise them.  Blocks. */
+int pf5d_bounds_stats_noalloc(int prec, const void* X, long ncell, double host_out5[5], void* stream);
+
 /* make_elastic_updater rank-1 path -- solver/elastic.py:147-263.  bhat_host:
  * kernel on the rfft half grid (k_shape) in J/m^3; l, X0: rank-1 vector and
  * reference composition.  apply(): mu[c*mstride + s] += psi[c] * l[s] with

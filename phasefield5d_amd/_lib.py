@@ -43,6 +43,7 @@ _SIGS = {
     "pf5d_update": (i32, [i32, vp, vp, i64, f64, vp]),
     "pf5d_clamp_simplex": (i32, [i32, vp, i64, vp]),
     "pf5d_bounds_stats": (i32, [i32, vp, i64, P_f64, vp]),
+    "pf5d_bounds_stats_noalloc": (i32, [i32, vp, i64, P_f64, vp]),
     "pf5d_elastic_create": (i32, [i32, i32, P_i64, P_f64, P_f64, P_f64, C.POINTER(vp)]),
     "pf5d_elastic_apply": (i32, [vp, vp, i64, vp, vp]),
     "pf5d_elastic_psi": (vp, [vp]),

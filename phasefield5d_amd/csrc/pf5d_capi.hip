@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -120,7 +121,7 @@ struct Scratch {
     return p;
   }
 };
-static Scratch& scratch() {
+static Scratch& grow_scratch() {
   static thread_local Scratch s;
   return s;
 }
@@ -1867,7 +1868,7 @@ int pf5d_field_stats(int prec, const void* A, long ncell, long astride, int ncom
     check_prec(prec);
     require(ncell > 0 && ncomp >= 1 && ncomp <= 8 && astride >= ncomp && out, "field_stats: bad arguments");
     const unsigned grid = (unsigned)std::min<long>(nblocks(ncell, 256), 4L * devinfo().cus);
-    double* d = (double*)scratch().get((size_t)(64 + grid * 24) * sizeof(double));
+    double* d = (double*)grow_scratch().get((size_t)(64 + grid * 24) * sizeof(double));
     double *acc = d, *mean = d + 40, *part = d + 64, *qpart = part + (size_t)grid * 16;
     hipStream_t st = S(stream);
     hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(64), 0, st, acc);
@@ -1922,7 +1923,7 @@ int pf5d_field_dot(int prec, const void* A, const void* B, long ncell, int ncomp
     check_prec(prec);
     require(ncell > 0 && ncomp >= 1 && ncomp <= 8 && out, "field_dot: bad arguments");
     const unsigned grid = (unsigned)std::min<long>(nblocks(ncell, 256), 4L * devinfo().cus);
-    double* part = (double*)scratch().get((size_t)grid * 8 * sizeof(double));
+    double* part = (double*)grow_scratch().get((size_t)grid * 8 * sizeof(double));
     hipStream_t st = S(stream);
     if (prec == PF5D_F64)
       hipLaunchKernelGGL(k_dot<double>, dim3(grid), dim3(256), 0, st, (const double*)A, (const double*)B, ncell,
@@ -1945,7 +1946,7 @@ int pf5d_field_dot(int prec, const void* A, const void* B, long ncell, int ncomp
 int pf5d_abs_max(int prec, const void* a, long n, double* host_result, void* stream) {
   return guarded([&] {
     check_prec(prec);
-    void* d = scratch().get(16);
+    void* d = grow_scratch().get(16);
     const unsigned grid = (unsigned)std::min<long>(nblocks(n, 256), 4L * devinfo().cus);
     double r = 0;
     if (prec == PF5D_F64) {
@@ -1994,28 +1995,84 @@ int pf5d_clamp_simplex(int prec, void* X, long ncell, void* stream) {
   });
 }
 
+}  // extern "C"
+
+// Blocks of the bounds statistics: one per 256 cells, at most two per CU.
+static unsigned bounds_grid(long ncell) { return (unsigned)std::min<long>(nblocks(ncell, 256), 2L * devinfo().cus); }
+
+// Bounds statistics of X with the per-block partials in device buffer d (grid * 5
+// doubles) and their host copy in h (as many): every block writes its own five
+// partials, so nothing needs clearing; the host folds them in block order.
+static void bounds_stats_into(int prec, const void* X, long ncell, double out5[5], hipStream_t st, double* d,
+                              double* h, unsigned grid) {
+  if (prec == PF5D_F64)
+    hipLaunchKernelGGL(k_oob_partial<double>, dim3(grid), dim3(256), 0, st, (const double*)X, ncell, d);
+  else
+    hipLaunchKernelGGL(k_oob_partial<float>, dim3(grid), dim3(256), 0, st, (const float*)X, ncell, d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h, d, (size_t)grid * 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double v[5] = {0, -1e300, 1e300, 1e300, -1e300};
+  for (unsigned b = 0; b < grid; ++b) {
+    v[0] += h[b * 5];
+    v[1] = std::max(v[1], h[b * 5 + 1]);
+    v[2] = std::min(v[2], h[b * 5 + 2]);
+    v[3] = std::min(v[3], h[b * 5 + 3]);
+    v[4] = std::max(v[4], h[b * 5 + 4]);
+  }
+  std::memcpy(out5, v, sizeof(v));
+}
+
+// Fixed device scratch of pf5d_bounds_stats_noalloc: the per-block partials of
+// up to NOALLOC_MAX_BLOCKS blocks (40 KiB), one copy per GPU for the life of the
+// process.  Host code never passes `scratch` itself to a kernel (in host code it
+// names a host-side stand-in); it uses the device address that
+// hipGetSymbolAddress returns for the current device, kept in scratch_dev.
+constexpr unsigned NOALLOC_MAX_BLOCKS = 1024;
+static __device__ __align__(256) double scratch[NOALLOC_MAX_BLOCKS * 5];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+static double* scratch_dev() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  require(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "bounds_stats_noalloc: device index out of range", PF5D_EDEVICE);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Pf5dError(PF5D_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
+extern "C" {
+
 int pf5d_bounds_stats(int prec, const void* X, long ncell, double out5[5], void* stream) {
   return guarded([&] {
     check_prec(prec);
-    const unsigned grid = (unsigned)std::min<long>(nblocks(ncell, 256), 2L * devinfo().cus);
-    double* d = (double*)scratch().get(grid * 5 * sizeof(double));
-    if (prec == PF5D_F64)
-      hipLaunchKernelGGL(k_oob_partial<double>, dim3(grid), dim3(256), 0, S(stream), (const double*)X, ncell, d);
-    else
-      hipLaunchKernelGGL(k_oob_partial<float>, dim3(grid), dim3(256), 0, S(stream), (const float*)X, ncell, d);
-    HIPCHK(hipGetLastError());
+    const unsigned grid = bounds_grid(ncell);
+    double* d = (double*)grow_scratch().get(grid * 5 * sizeof(double));
     std::vector<double> h(grid * 5);
-    HIPCHK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, S(stream)));
-    HIPCHK(hipStreamSynchronize(S(stream)));
-    double v[5] = {0, -1e300, 1e300, 1e300, -1e300};
-    for (unsigned b = 0; b < grid; ++b) {
-      v[0] += h[b * 5];
-      v[1] = std::max(v[1], h[b * 5 + 1]);
-      v[2] = std::min(v[2], h[b * 5 + 2]);
-      v[3] = std::min(v[3], h[b * 5 + 3]);
-      v[4] = std::max(v[4], h[b * 5 + 4]);
-    }
-    std::memcpy(out5, v, sizeof(v));
+    bounds_stats_into(prec, X, ncell, out5, S(stream), d, h.data(), grid);
+  });
+}
+
+int pf5d_bounds_stats_noalloc(int prec, const void* X, long ncell, double out5[5], void* stream) {
+  return guarded([&] {
+    check_prec(prec);
+    require(X && out5, "bounds_stats_noalloc: null argument");
+    const unsigned grid = bounds_grid(ncell);
+    if (grid > NOALLOC_MAX_BLOCKS)
+      throw Pf5dError(PF5D_EINVAL, "bounds_stats_noalloc: " + std::to_string(grid) + " blocks of partials exceed the " +
+                                       std::to_string(NOALLOC_MAX_BLOCKS) + " the fixed scratch holds");
+    double h[NOALLOC_MAX_BLOCKS * 5];
+    bounds_stats_into(prec, X, ncell, out5, S(stream), scratch_dev(), h, grid);
   });
 }
 
