@@ -20,7 +20,7 @@ import numpy as np
 
 from .problem import build_problem
 from .solver.config import parse_args_to_config
-from .solver.io import (SaveSchedule, finalize_traces, init_traces, make_path, save_current_state,  # noqa: F401
+from .solver.io import (SaveSchedule, finalize_traces, init_traces, make_problem_path, save_current_state,  # noqa: F401
                         update_traces)
 
 
@@ -40,7 +40,7 @@ def run(argv=None):
     pb = build_problem(cfg, verbose=True)
     print(f"setup: shape {pb.shape}, dx {pb.dx:.3e} m, ctld {pb.ctld:.4e} s, dt0 {pb.dt0:.3e}, "
           f"dt_cfl {pb.dt_cfl:.3e} ({wall.perf_counter() - t0:.1f} s)", flush=True)
-    run_dir = make_path(cfg, pb) if cfg.save_outputs else None
+    run_dir = make_problem_path(cfg, pb) if cfg.save_outputs else None
 
     def save_state(step, dt, t, X, postfix=""):
         if run_dir is not None:

@@ -111,3 +111,12 @@ def make_elastic_updater(cfg, linear_elastic_coupling_matrix, elastic_kernel_r, 
     prec = _lib.F32 if np.dtype(field_dtype) == np.float32 else _lib.F64
     X0 = np.asarray(cfg.initial_composition, dtype=np.float64)
     return _DeviceElastic(prec, spatial, elastic_kernel_r, l_vec.astype(field_dtype).astype(np.float64), X0)
+
+
+def calculate_elastic_potential(reference_composition, composition_field, linear_elastic_coupling_matrix,
+                                elastic_kernel_r, work_real=None, work_k=None, work_k_coupled=None, out=None,
+                                fft_workers=1):
+    """The general (non-rank-1) elastic path of elastic.py:114-140.  The Vegard coupling is
+    always rank 1, so the reference driver never reaches it; it is not built here."""
+    raise NotImplementedError("the general (non-rank-1) elastic potential is not implemented; "
+                              "make_elastic_updater's rank-1 path covers Vegard coupling")

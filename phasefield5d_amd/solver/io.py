@@ -91,40 +91,95 @@ class SaveSchedule:
         return max(0, n)
 
 
-def make_path(cfg, pb, root=None):
-    """Create the run directory tree and metadata.json (io.py:114-198); returns its path."""
-    from .system import get_system_dimensions
-    root = root or cfg.results_root
+def should_save(step, ctld, time, total_steps, *, early_frames=100, burst_duration_ctld=100.0, burst_frames=800,
+                tail_dt_ctld=10.0, snapshot_factor=10, save_every_n_steps_data=50_000,
+                save_every_n_steps_snap=100_000):
+    """io.py:42-109: (save_data, save_snapshot) for loop index ``step`` at ``time``.
+
+    Stateful like the reference: step 0 (or the first call) starts a new schedule,
+    kept in the function attribute ``should_save._state`` (a dict with the phase
+    ends, the data / snapshot spacings of each phase and the next save times), and
+    later calls advance it.  The spacing rules are those of ``SaveSchedule``.
+    """
+    st = getattr(should_save, "_state", None)
+    if st is None or step == 0:
+        sched = SaveSchedule(ctld, total_steps, early_frames, burst_duration_ctld, burst_frames, tail_dt_ctld,
+                             snapshot_factor, save_every_n_steps_data, save_every_n_steps_snap)
+        should_save._state = vars(sched)
+        should_save._sched = sched
+        return True, True
+    sched = should_save._sched
+    # the reference reads its limits from the state it made at step 0; the call's
+    # own total_steps and every-N values apply from this call on
+    sched.total = total_steps
+    sched.every_data, sched.every_snap = save_every_n_steps_data, save_every_n_steps_snap
+    return sched(step, time)
+
+
+def _results_root(cfg):
+    return getattr(cfg, "results_root", None) or os.path.join(os.getcwd(), "results")
+
+
+def make_path(cfg, hessian_max, mobility_max, number_of_cells, cell_size, system_length, ctld, wavenumber_max,
+              wavelength_max, time_increment, root=None):
+    """io.py:114-198 (make_path :193): create the run directory tree and metadata.json; return its path.
+
+    Same keyword arguments as the reference; ``root`` (added) overrides the results
+    root (``cfg.results_root`` if the config has one, else ./results).
+    """
+    root = root or _results_root(cfg)
     label = get_composition_label(cfg.initial_composition, unicode=False, elements=cfg.elements)
     sub = os.path.join(root, f"{label}_at_{cfg.temperature}_{cfg.system_dim}dim",
                        get_model_tag(cfg.atomic_radius_tag, cfg.include_cubic_anisotropy, cfg.direction,
                                      cfg.system_dim))
     stamp = datetime.datetime.now().strftime("%Y%m%d_%H%M%S")
-    cells = "x".join(str(n) for n in pb.shape)
-    name = (f"cells{cells}_dx{pb.dx:.0e}_fluc{cfg.fluctuation:.0e}_kappa{cfg.kappa_value:.0e}"
-            f"_steps{cfg.total_timesteps}_dt{pb.dt0:.0e}s_{stamp}")
+    cells = "x".join(str(n) for n in number_of_cells)
+    name = (f"cells{cells}_dx{cell_size:.0e}_fluc{cfg.fluctuation:.0e}_kappa{cfg.kappa_value:.0e}"
+            f"_steps{cfg.total_timesteps}_dt{time_increment:.0e}s_{stamp}")
     run_dir = os.path.abspath(os.path.join(sub, name))
     for d in ("snapshots", "data"):
         os.makedirs(os.path.join(run_dir, d), exist_ok=True)
-    wavelength, _, system_length, _ = get_system_dimensions(pb.k_max, ppw=cfg.ppw, mw=cfg.mw, dim=cfg.system_dim)
     meta = {
         "elements": cfg.elements, "temperature": cfg.temperature,
         "initial_composition": np.asarray(cfg.initial_composition).tolist(),
-        "hessian_max": pb.hessian_max, "mobility_max": pb.mobility_max, "system_dim": cfg.system_dim,
+        "hessian_max": hessian_max, "mobility_max": mobility_max, "system_dim": cfg.system_dim,
         "include_cubic_anisotropy": cfg.include_cubic_anisotropy, "direction": np.asarray(cfg.direction).tolist(),
-        "atomic_radius": cfg.atomic_radius_tag, "mw": cfg.mw, "ppw": cfg.ppw, "number_of_cells": list(pb.shape),
-        "cell_size": pb.dx, "system_length": system_length, "fluctuation": cfg.fluctuation,
-        "kappa": cfg.kappa_value, "total_timesteps": cfg.total_timesteps, "ctld": pb.ctld,
-        "wavenumber_max": pb.k_max, "wavelength_max": wavelength, "time_steps_per_ctld": cfg.steps_per_ctld,
-        "time_increment": pb.dt0, "safety": cfg.safety, "lower_limit": cfg.lower_limit,
-        "upper_limit": cfg.upper_limit, "time_linear_end_multiplier": cfg.time_linear_end_multiplier,
+        "atomic_radius": cfg.atomic_radius_tag, "mw": cfg.mw, "ppw": cfg.ppw,
+        "number_of_cells": list(number_of_cells), "cell_size": cell_size, "system_length": system_length,
+        "fluctuation": cfg.fluctuation, "kappa": cfg.kappa_value, "total_timesteps": cfg.total_timesteps,
+        "ctld": ctld, "wavenumber_max": wavenumber_max, "wavelength_max": wavelength_max,
+        "time_steps_per_ctld": cfg.steps_per_ctld, "time_increment": time_increment, "safety": cfg.safety,
+        "lower_limit": cfg.lower_limit, "upper_limit": cfg.upper_limit,
+        "time_linear_end_multiplier": cfg.time_linear_end_multiplier,
         "early_stage_frames": cfg.early_stage_frames, "late_burst_duration_ctld": cfg.late_burst_duration_ctld,
         "late_burst_frames": cfg.late_burst_frames, "late_tail_frames": cfg.late_tail_frames,
         "threads": cfg.threads, "timestamp": stamp, "run_dir": run_dir,
     }
     with open(os.path.join(run_dir, "metadata.json"), "w") as f:
         json.dump(meta, f, indent=4)
+    print(f"Simulation path created at:\n → {run_dir}")
     return run_dir
+
+
+def make_problem_path(cfg, pb, root=None):
+    """``make_path`` with the set-up values taken from a built problem (``problem.build_problem``)."""
+    from .system import get_system_dimensions
+    wavelength, _, system_length, _ = get_system_dimensions(pb.k_max, ppw=cfg.ppw, mw=cfg.mw, dim=cfg.system_dim)
+    return make_path(cfg=cfg, hessian_max=pb.hessian_max, mobility_max=pb.mobility_max, number_of_cells=pb.shape,
+                     cell_size=pb.dx, system_length=system_length, ctld=pb.ctld, wavenumber_max=pb.k_max,
+                     wavelength_max=wavelength, time_increment=pb.dt0, root=root)
+
+
+_FIGURE_NOTE = []
+
+
+def save_snapshot_figure(*args, **kwargs):
+    """io.py:223 draws matplotlib snapshot PNGs.  Figures are not part of this build:
+    the call only says so, once, and the state files and traces carry the data."""
+    if not _FIGURE_NOTE:
+        _FIGURE_NOTE.append(True)
+        print("save_snapshot_figure: snapshot figures are not drawn by phasefield5d_amd "
+              "(the data/ npz states hold the fields)")
 
 
 def save_current_state(path, timestep, time_increment, time, current_composition, postfix=""):

@@ -70,6 +70,17 @@ def apply_fluctuations(reference_composition, size, fluctuation, seed=None):
     return out
 
 
+def snap_to_grid_floor(comp, step=0.01):
+    """system.py:64-73: rows of compositions (n, k) clipped to [0, 1], rows summing above
+    1 scaled back onto the simplex, then floored to the table grid of spacing ``step``
+    (with a 1e-9 guard so values already on a node stay there).  Host set-up helper."""
+    c = np.clip(np.array(comp, dtype=float), 0.0, 1.0)
+    tot = c.sum(axis=-1, keepdims=True)
+    over = tot[:, 0] > 1.0
+    c[over] = c[over] / tot[over]
+    return np.floor(c * (1.0 / step) + 1e-9).astype(int) / (1.0 / step)
+
+
 def get_time_increment(ctld, steps_per_ctld=100):
     return round_down_to_first_nonzero(ctld / steps_per_ctld)
 
