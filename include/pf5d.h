@@ -199,8 +199,9 @@ int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups);
 /* average device time of one launch of a stage (HIP events on the engine stream):
  * which = 0 stencil stage K1 (either form), 1 the FFT passes, 2 dt controller, 3 update
  * kernel in its statistics-only mode, 4 / 5 the potentials / flux pass of the two-pass
- * stencil form (when selected).  The stencil is idempotent, so timing it leaves the
- * state of the step unchanged. */
+ * stencil form (when selected).  The timed launches rewrite the dX/dt and psi fields
+ * from the current state; the call then restores what the next step reads (the
+ * max|dX/dt| accumulator, the FP64 statistics), so the trajectory is unchanged. */
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms);
 
 /* per-stage device time of whole steps of a one-group engine with the marching two-pass

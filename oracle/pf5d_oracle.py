@@ -22,9 +22,12 @@ Pinning of each function (tests/test_oracle_pins.py):
                            sum over cells = 0 (telescoping).
   elastic_psi              1D: B constant for k != 0 -> psi = B (s - mean s);
                            plane wave -> B(k) cos(k.x); B(0) = 0.
-  abs_max / adaptive dt /  closed-form scalar rules on an exhaustive input grid.
-  clamp / out_of_bounds
-  apply_fluctuations       identical draw sequence under np.random.seed.
+  misfit_scalar /          X = X0 + a v cos(k.x) -> mu += B(k) a (l.v) cos(k.x) l
+  elastic_update           (2D in-plane and 3D); the misfit is linear with slope l_j.
+  abs_max / adaptive dt /  closed-form scalar rules on an exhaustive input grid;
+  clamp / out_of_bounds    each hard bounds branch alone, soft-only cases.
+  apply_fluctuations       a hand-built draw: the seeded legacy generator's doubles
+                           one at a time, five per cell, mean over all five removed.
 """
 import numpy as np
 import scipy.fft as sfft

@@ -1064,6 +1064,9 @@ struct Engine : EngineBase {
       }
       fill(s);
       HIPCHK(hipMemset(s.dX, 0, ncell * 4 * sizeof(T)));
+      // psi is written by the first step's FFT passes; zeroed so that nothing
+      // reads uninitialised memory before (a stencil timed right after add)
+      if (s.psi) HIPCHK(hipMemset(s.psi, 0, ncell * sizeof(T)));
     } catch (...) {
       free_sim(s);
       throw;
@@ -1619,7 +1622,17 @@ struct Engine : EngineBase {
     HIPCHK(hipEventRecord(a, stream));
     for (int i = 0; i < reps; ++i) one();
     HIPCHK(hipEventRecord(b, stream));
+    // the timed stencil folded max|dX/dt| of the current fields into the
+    // accumulator the next step's dt controller reads, and a timed controller
+    // reduced whatever partials the update pass last left: restore both (the
+    // accumulator to zero, the statistics to those of the current fields)
+    if (nk2) {
+      launch_k2(1);
+      launch_dt(0);
+      stats_stale = false;
+    }
     HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipStreamSynchronize(stream));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);

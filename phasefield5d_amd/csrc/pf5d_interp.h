@@ -40,6 +40,10 @@ struct TableView {
   int cube_s;      // cubes with lower-corner index sum <= cube_s are in it
 };
 
+// nearest table row for points past the linear region (defined below)
+template <typename T>
+PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]);
+
 // Cube table: for every hypercube of the linear region (lower corner i with
 // i0+i1+i2+i3 <= S, S = num - 5) the 16 corners' nine properties, stored in
 // the order the interpolation reads them, CUBE_VALUES values per cube: for
@@ -430,39 +434,45 @@ PF5D_DEV NodeBox stage_node_box(const TableView<T>& tab, int v[8], T* box, int* 
   return nb;
 }
 
-// Nearest valid table node (the reference's KD-tree query on the table rows,
-// interpolation.py:203-209): the squared distance is separable, so the nearest
-// grid node is the per-axis nearest, one of {floor(u), floor(u) + 1} on every
-// axis.  Those 16 nodes are scanned first (in the order and with the tie rule of
-// the full scan below); when the per-axis nearest node is a valid one (inside
-// the simplex) it is the answer.  Only otherwise -- the point lies beyond the
-// simplex face, whose nearest nodes can be further out -- are the 4^4 nodes
-// around it scanned.
+// Nearest table row (the reference's KD-tree query on the table rows,
+// interpolation.py:203-209).  Rows are the nodes of the dense grid that hold
+// values: inside the simplex and not missing from the table (build_4d_grid
+// leaves NaN where a row is absent, interpolation.py:30-34).
+//
+// With u = X/h and f = floor(u) per axis, the boxes B_R = [f - R, f + 1 + R]^4
+// are scanned for R = 0, 1, 2, 4, 8, ...  A node outside B_R is at least (R + 1) h
+// away on some axis, so once the best squared distance found is at most
+// ((R + 1) h)^2 no node outside can be nearer and the search stops.  B_0 holds
+// the per-axis nearest node, within h of the point: when it is a row (the usual
+// case) the first box settles it.  Nodes are visited in lexicographic order and
+// a candidate replaces the best only when strictly nearer, so equidistant rows
+// resolve to the first in that order at every R.  Missing rows are skipped by
+// reading a candidate's first value only when it would become the best.
 template <typename T>
-PF5D_DEV void nearest_scan(const T X[4], int num, double h, const int lo[4], int a0, int a1, double& best,
-                           long& bidx) {
+PF5D_DEV void nearest_scan_box(const TableView<T>& tv, const double X[4], const int lo[4], const int hi[4],
+                               double& best, long& bidx) {
+  constexpr int NP = TablePad<T>::value;
+  const int num = tv.num;
+  const double h = tv.step;
 #pragma unroll 1
-  for (int a = a0; a < a1; ++a) {
-    int n0 = lo[0] + a;
-    if (n0 < 0 || n0 >= num) continue;
+  for (int n0 = lo[0]; n0 <= hi[0]; ++n0) {
+    const double d0 = X[0] - n0 * h;
 #pragma unroll 1
-    for (int b = a0; b < a1; ++b) {
-      int n1 = lo[1] + b;
-      if (n1 < 0 || n1 >= num || n0 + n1 > num - 1) continue;
+    for (int n1 = lo[1]; n1 <= hi[1] && n0 + n1 <= num - 1; ++n1) {
+      const double d1 = X[1] - n1 * h;
 #pragma unroll 1
-      for (int c = a0; c < a1; ++c) {
-        int n2 = lo[2] + c;
-        if (n2 < 0 || n2 >= num || n0 + n1 + n2 > num - 1) continue;
+      for (int n2 = lo[2]; n2 <= hi[2] && n0 + n1 + n2 <= num - 1; ++n2) {
+        const double d2 = X[2] - n2 * h;
 #pragma unroll 1
-        for (int e = a0; e < a1; ++e) {
-          int n3 = lo[3] + e;
-          if (n3 < 0 || n3 >= num || n0 + n1 + n2 + n3 > num - 1) continue;
-          double d0 = (double)X[0] - n0 * h, d1 = (double)X[1] - n1 * h;
-          double d2 = (double)X[2] - n2 * h, d3 = (double)X[3] - n3 * h;
-          double dd = d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
+        for (int n3 = lo[3]; n3 <= hi[3] && n0 + n1 + n2 + n3 <= num - 1; ++n3) {
+          const double d3 = X[3] - n3 * h;
+          const double dd = d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
           if (dd < best) {
-            best = dd;
-            bidx = (((long)n0 * num + n1) * num + n2) * num + n3;
+            const long row = (((long)n0 * num + n1) * num + n2) * num + n3;
+            if (!__builtin_isnan((double)gld<T>(tv.data + row * NP))) {
+              best = dd;
+              bidx = row;
+            }
           }
         }
       }
@@ -475,28 +485,35 @@ PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
   constexpr int NP = TablePad<T>::value;
   const int num = tv.num;
   const double h = tv.step;
-  int lo[4], near_sum = 0;
-  bool near_in = true;
+  double x[4];
+  int f[4];
+  bool finite = true;
 #pragma unroll
   for (int d = 0; d < 4; ++d) {
-    double u = (double)X[d] / h;
-    int f = (u <= 0.0) ? 0 : (int)u;
-    if (f > num - 1) f = num - 1;
-    lo[d] = f - 1;
-    // per-axis nearest of the candidates f, f + 1 (ties to f, as the scan keeps the first)
-    const double df = (double)X[d] - f * h, dg = (double)X[d] - (f + 1) * h;
-    const int r = (f + 1 < num && dg * dg < df * df) ? f + 1 : f;
-    near_sum += r;
-    near_in = near_in && r >= 0;
+    x[d] = (double)X[d];
+    finite = finite && __builtin_isfinite(x[d]);
+    const double u = x[d] / h;
+    f[d] = (u <= 0.0) ? 0 : (u >= num - 1) ? num - 1 : (int)u;
   }
-  near_in = near_in && near_sum <= num - 1;
   double best = 1e300;
   long bidx = -1;
-  nearest_scan<T>(X, num, h, lo, 1, 3, best, bidx);
-  if (!near_in) {
-    best = 1e300;
-    bidx = -1;
-    nearest_scan<T>(X, num, h, lo, 0, 4, best, bidx);
+  if (finite) {
+#pragma unroll 1
+    for (int R = 0;; R = (R == 0) ? 1 : 2 * R) {
+      int lo[4], hi[4];
+      bool whole = true;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        lo[d] = max(0, f[d] - R);
+        hi[d] = min(num - 1, f[d] + 1 + R);
+        whole = whole && lo[d] == 0 && hi[d] == num - 1;
+      }
+      best = 1e300;
+      bidx = -1;
+      nearest_scan_box<T>(tv, x, lo, hi, best, bidx);
+      const double reach = (R + 1) * h;
+      if ((bidx >= 0 && best <= reach * reach) || whole) break;
+    }
   }
   if (bidx < 0) {
 #pragma unroll

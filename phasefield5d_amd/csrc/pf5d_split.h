@@ -643,8 +643,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
     if (tid == 0) {
       // the count depends on the values the maxima's atomics returned, so it is
       // made after they were performed (at device scope, where the controller's
-      // exchange reads them); no cache write-back is needed for that
-      const unsigned one = 1u + (folded == ~0ull ? 1u : 0u);  // the bits of a NaN with the sign set: never stored
+      // exchange reads them); no cache write-back is needed for that.  The empty
+      // asm hides folded's value from the compiler: without it, the compare below
+      // (never true, since a stored maximum is a non-negative value whose bits
+      // are not all ones) could be folded away, the maxima's atomics would lose
+      // their returns and nothing would order them before the count.
+      asm volatile("" : "+v"(folded));
+      const unsigned one = 1u + (folded == ~0ull ? 1u : 0u);
       last = atomicAdd(done, one) == gridDim.x - 1;
     }
     __syncthreads();

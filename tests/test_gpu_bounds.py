@@ -77,3 +77,50 @@ def test_statistics_current_after_unchecked_steps(dev):
     tot = X.sum(axis=-1)
     assert st["sum_min"] == pytest.approx(tot.min(), rel=1e-14) and st["sum_max"] == pytest.approx(tot.max(), rel=1e-14)
     e.close()
+
+
+# one cell per case, the rest of the field at X0 = 0.1 per component (system.py:12-53)
+CASES = [
+    ("sum > 1 + tol_hard", [0.5, 0.3, 0.2, 0.0015], True, True),
+    ("sum > 1 + tol_soft only", [0.5, 0.3, 0.2, 0.0005], False, True),
+    ("sum < -tol_hard", [-0.0009, -0.0009, 0.0004, 0.0], True, True),
+    ("component < -tol_hard", [-0.0015, 0.1, 0.1, 0.1], True, True),
+    ("component < -tol_soft only", [-0.0002, 0.1, 0.1, 0.1], False, True),
+    ("component > 1 + tol_hard", [1.0015, -0.0009, -0.0009, -0.0009], True, True),
+    ("non-finite", [np.nan, 0.1, 0.1, 0.1], True, False),
+    ("inside", [0.1, 0.1, 0.1, 0.1], False, False),
+]
+
+
+@pytest.mark.parametrize("interval", [1, 100])
+@pytest.mark.parametrize("name,cell,hard,soft", CASES, ids=[c[0] for c in CASES])
+def test_bounds_branches_vs_oracle(dev, name, cell, hard, soft, interval):
+    """Every hard branch alone, the soft branch alone, NaN and a clean field: the drop-in
+    out_of_bounds (device reduction) agrees with the oracle's, and an engine holding the
+    field stops before its first step exactly when the field is hard out of bounds and
+    flags the soft violations while stepping on (checks at step 0 for both intervals)."""
+    from phasefield5d_amd.engine import ST_ACTIVE, ST_HARD_OOB, ST_SOFT_OOB, Engine
+    from phasefield5d_amd.solver.system import out_of_bounds
+    from oracle import pf5d_oracle as O
+    pb = problem(2, 16, elastic=False)
+    X = np.full((16, 16, 4), 0.1)
+    X[7, 3] = cell
+    assert O.out_of_bounds(X) is hard
+    assert out_of_bounds(X) is hard
+    assert out_of_bounds(torch_tensor(X)) is hard
+    pb.X_init = X
+    e = Engine("fp64")
+    s = e.add_problem(pb, oob_interval=interval)
+    e.step(3)
+    st = e.state(s)
+    e.close()
+    if hard:
+        assert st["status"] & ST_HARD_OOB and not st["status"] & ST_ACTIVE and st["steps"] == 0
+    else:
+        assert st["status"] & ST_ACTIVE and not st["status"] & ST_HARD_OOB and st["steps"] == 3
+    assert bool(st["status"] & ST_SOFT_OOB) is soft
+
+
+def torch_tensor(X):
+    import torch
+    return torch.from_numpy(X).cuda()

@@ -52,6 +52,56 @@ def test_interpolator_bitwise(dev, dim, n):
     assert np.array_equal(P.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_interpolator_table_with_missing_rows(dev, prec):
+    """A table with rows missing: 3 % of the nodes at random, plus the nearest node and
+    its neighbours around some fallback points, so the nearest present row lies one to
+    three grid steps out.  build_4d_grid leaves NaN there (interpolation.py:30-34); the
+    fallback is the nearest present row, as cKDTree over the rows gives it (:203-209);
+    linear cells with a missing corner stay NaN as in the reference.  Also points far
+    past the simplex face (sum X = 1.3), whose nearest rows are many steps away."""
+    from phasefield5d_amd.solver.interpolation import build_4d_grid, build_calphad_kdtree, interpolator_nb
+    from phasefield5d_amd.problem import table_rows
+    pb = problem(2, 32)
+    rows = table_rows(pb.cfg)
+    h = pb.cfg.resolution
+    rng = np.random.default_rng(5)
+    fallback = np.array([[0.31, 0.30, 0.21, 0.15], [0.331, 0.309, 0.2013, 0.1604], [0.52, 0.2, 0.15, 0.11],
+                         [0.5, 0.5, 0.3, 0.0], [0.9, 0.2, 0.1, 0.1], [0.24, 0.26, 0.24, 0.2201]])
+    idx = np.rint(rows[:, :4] / h).astype(int)
+    drop = rng.random(len(rows)) < 0.03
+    for p in fallback[:3]:
+        near = np.rint(p / h).astype(int)
+        drop |= np.abs(idx - near).max(axis=1) <= (1 if p[0] < 0.32 else 2)
+    kept = rows[~drop]
+    print(f"[holes] {prec}: {drop.sum()} of {len(rows)} rows removed", flush=True)
+    _, grid = build_4d_grid(kept, h)
+    tree, values = build_calphad_kdtree(kept)
+    # equidistant rows are resolved by the KD-tree's traversal order in the reference,
+    # which nothing specifies (the device takes the first in node order): no ties here
+    d2, _ = tree.query(fallback, k=2)
+    assert np.all(d2[:, 1] - d2[:, 0] > 1e-9 * h), d2 / h
+    X = developed_state(pb, amp=0.08)
+    Xf = X.reshape(-1, 4)
+    Xf[: len(fallback)] = fallback
+    Xf[len(fallback):len(fallback) + 40] = rng.uniform(0.18, 0.24, size=(40, 4))  # linear, some hit holes
+    dtype = np.float64 if prec == "fp64" else np.float32
+    Xp = X.astype(dtype)
+    out = interpolator_nb(Xp, grid, h, tree, values).astype(np.float64)
+    if prec == "fp64":
+        ref = O.interpolator(Xp, grid, h, tree, values)
+        fb = (((Xp[..., 0] + Xp[..., 1]) + Xp[..., 2]) + Xp[..., 3]) > O.linear_threshold(h)
+        assert fb.sum() >= len(fallback) and np.isnan(ref[~fb]).any()
+        assert np.array_equal(out, ref, equal_nan=True)
+    else:
+        # the device decides in FP32 which cells fall back; those take the nearest
+        # present row (searched in FP64 from the FP32 point), stored in FP32
+        fb = (((Xp[..., 0] + Xp[..., 1]) + Xp[..., 2]) + Xp[..., 3]) > np.float32(O.linear_threshold(h))
+        _, nn = tree.query(Xp[fb].astype(np.float64))
+        assert fb.sum() >= len(fallback)
+        assert np.array_equal(out[fb], values[nn].astype(np.float32).astype(np.float64))
+
+
 @pytest.mark.parametrize("dim,n", [(1, 160), (2, 48), (3, 24)])
 def test_laplacian_and_divergence_bitwise(dev, dim, n):
     from phasefield5d_amd.solver.fluxes import get_divergence_from_mu_function

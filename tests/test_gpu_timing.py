@@ -53,3 +53,26 @@ def test_stage_timing_rejects_groups(dev):
     with pytest.raises(ValueError, match="more than one group"):
         e.time_step_stages(2)
     e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_kernel_timing_leaves_trajectory_unchanged(dev, precision):
+    """Timing single stages between steps (and right after add, before any step) must not
+    change what the steps compute: the timed stencil's max|dX/dt| and a timed dt
+    controller's reduction would otherwise reach the next step's dt."""
+    pb = problem(3, 24, seed=9)
+    n = 6
+    X_ref, st_ref, _ = _run(pb, precision, lambda e: e.step(n))
+
+    def timed(e):
+        for which in ("k1", "k1_potentials", "k1_flux", "dt", "fft", "k2prime"):
+            e.time_kernel(which, reps=3)
+        e.step(3)
+        for which in ("k1", "dt", "k1_flux"):
+            e.time_kernel(which, reps=2)
+        e.step(n - 3)
+
+    X, st, _ = _run(pb, precision, timed)
+    assert st["steps"] == n and st["dt"] == st_ref["dt"] and st["t"] == st_ref["t"]
+    assert np.array_equal(st["mass"], st_ref["mass"])
+    assert np.array_equal(X, X_ref)
