@@ -198,7 +198,8 @@ def main():
         et = Engine("fp32")
         et.add_problem(pb)
         et.set_tiling(info["variant"], info["cx"])
-        et.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
+        et.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"],
+                   wave=info["fft_wave"])
         et.step(args.warmup)
         et.sync()
         in_step = et.time_step_stages(min(max(args.steps, 50), 4096))
@@ -218,7 +219,8 @@ def main():
         pbd = dataclasses.replace(pb, X_init=developed_field(pb, amp=0.05, seed=4))
         sd_ = ed.add_problem(pbd)
         ed.set_tiling(info["variant"], info["cx"])
-        ed.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
+        ed.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"],
+                   wave=info["fft_wave"])
         ed.step(5)
         ed.sync()
         t0 = time.perf_counter()
@@ -240,7 +242,8 @@ def main():
         for _ in range(nb):
             eb.add_problem(pb)
         eb.set_tiling(info["variant"], info["cx"])
-        eb.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"])
+        eb.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"],
+                   wave=info["fft_wave"])
         eb.step(8)
         eb.sync()
         kb = 48

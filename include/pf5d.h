@@ -171,6 +171,12 @@ int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt);
  * passes only (the default).  Where the plane passes need more LDS than a block
  * has, the line passes are used. */
 int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts, int nt);
+/* Wave-resident FFT passes for 3D fields whose plane (y and z lengths) or x length
+ * is 128: each 128-point line transformed in the registers of 8 threads of one
+ * wavefront.  mode is a mask of the passes that use them: 1 the forward plane pass,
+ * 2 the x pass, 4 the inverse plane pass (the others run the Stockham plane or line
+ * passes above).  Defaults: 1 in FP32, 7 in FP64.  Every mode gives the same bits. */
+int pf5d_engine_set_fft_wave(pf5d_engine* e, int mode);
 /* enqueue nsteps (graph replay); does not block */
 int pf5d_engine_step(pf5d_engine* e, int nsteps);
 int pf5d_engine_sync(pf5d_engine* e);
@@ -190,8 +196,9 @@ void* pf5d_engine_stream(pf5d_engine* e);
 int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes);
 /* launch configuration of the current work lists, for profiling reports: [K1 items, K2 items,
  * FFT items, K1 grid, K1 LDS bytes, stencil variant, x chunk, FFT LDS budget bytes, FFT block
- * size, flux-pass grid, FFT plane parts in use (0: line passes only), FFT plane block size] */
-int pf5d_engine_launch_info(pf5d_engine* e, long out[12]);
+ * size, flux-pass grid, FFT plane parts in use (0: line passes only), FFT plane block size,
+ * mask of the FFT passes running wave-resident (pf5d_engine_set_fft_wave)] */
+int pf5d_engine_launch_info(pf5d_engine* e, long out[13]);
 /* number of simulation groups the engine steps one after another (each group's per-step
  * working set fits the GPU's last-level cache; 1: all simulations step together) */
 int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups);

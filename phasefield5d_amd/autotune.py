@@ -24,7 +24,7 @@ _lock = threading.Lock()
 
 def signature(engine):
     shapes = sorted(tuple(pb.shape) for pb, _ in engine.slots.values())
-    return f"v4:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
+    return f"v5:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
 
 
 def _load():
@@ -58,7 +58,8 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
         ent = cache[sig]
         engine.set_tiling(ent["variant"], ent["cx"])
         if "fft_kb" in ent:
-            engine.set_fft(ent["fft_kb"], ent["fft_nt"], ent.get("fft_parts", 0), ent.get("fft_plane_nt", 256))
+            engine.set_fft(ent["fft_kb"], ent["fft_nt"], ent.get("fft_parts", 0), ent.get("fft_plane_nt", 256),
+                           wave=ent.get("fft_wave", 0))
         return ent
     has3d = any(len(pb.shape) == 3 for pb, _ in engine.slots.values())
     results = []
@@ -86,17 +87,17 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
     # then the plane passes (parts per plane and block size) of 3D fields with
     # 128 x 128 planes, then the line passes again for what the planes leave
     if any(pb.Bhat is not None for pb, _ in engine.slots.values()):
-        def trial(kb, nt, parts, pnt):
+        def trial(kb, nt, parts, pnt, wave=0):
             try:
-                engine.set_fft(kb, nt, parts, pnt)
+                engine.set_fft(kb, nt, parts, pnt, wave=wave)
                 fms = engine.time_kernel("fft", reps=reps)
             except (ValueError, RuntimeError) as e:
                 if verbose:
-                    print(f"  fft {kb} KiB / {nt}, planes {parts} / {pnt}: skipped ({e})")
+                    print(f"  fft {kb} KiB / {nt}, planes {parts} / {pnt}, wave {wave}: skipped ({e})")
                 return None
             if verbose:
-                print(f"  fft {kb} KiB / {nt} threads, planes {parts} / {pnt}: {fms:.4f} ms")
-            return (fms, kb, nt, parts, pnt)
+                print(f"  fft {kb} KiB / {nt} threads, planes {parts} / {pnt}, wave {wave}: {fms:.4f} ms")
+            return (fms, kb, nt, parts, pnt, wave)
 
         budgets, nts = ((16,), (128, 256)) if quick else (fft_budgets, (128, 256, 512))
         plane_cfgs = ((2, 1024), (4, 1024)) if quick else tuple((p, n) for p in (2, 4, 8) for n in (256, 512, 1024))
@@ -111,9 +112,18 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
                     again = [r for r in (trial(kb, nt, best[3], best[4]) for kb in (16, 32, 64)
                                          for nt in (128, 256)) if r]
                     best = min(again + [best])
-            fms, kb, nt, parts, pnt = best
-            engine.set_fft(kb, nt, parts, pnt)
-            entry.update({"fft_kb": kb, "fft_nt": nt, "fft_parts": parts, "fft_plane_nt": pnt, "fft_ms": fms})
+            # the wave-resident passes of 128-point planes and x lines, where the mix has them
+            waves = []
+            for mode in ((1, 7) if quick else range(1, 8)):
+                r = trial(best[1], best[2], best[3], best[4], mode)
+                if r and engine.launch_info()["fft_wave"]:
+                    waves.append(r)
+            if waves and min(waves) < best:
+                best = min(waves)
+            fms, kb, nt, parts, pnt, wave = best
+            engine.set_fft(kb, nt, parts, pnt, wave=wave)
+            entry.update({"fft_kb": kb, "fft_nt": nt, "fft_parts": parts, "fft_plane_nt": pnt, "fft_wave": wave,
+                          "fft_ms": fms})
     with _lock:
         cache = _load()
         cache[sig] = entry

@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 UNITS = ["pf5d_capi.hip", "pf5d_kern_fft32.hip", "pf5d_kern_fft64.hip", "pf5d_kern_fftplane.hip",
-         "pf5d_kern_stencil.hip"]
+         "pf5d_kern_stencil.hip", "pf5d_kern_fftreg.hip"]
 OUT = os.path.join(HERE, "libpf5d.so")
 OBJDIR = os.path.join(HERE, "_obj")
 

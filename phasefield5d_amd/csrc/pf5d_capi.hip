@@ -337,6 +337,34 @@ static PlaneKernels<T> plane_kernels(int n, int parts, int nt) {
   return {};
 }
 
+// wave-resident passes of 128-point planes and x lines (pf5d_fftreg.h), one
+// set per precision; the engine's wave mode is a mask of the passes that use
+// them: 1 the forward plane pass, 2 the x pass, 4 the inverse plane pass
+constexpr int FFT_WAVE_ALL = 7;
+template <typename T>
+struct WaveKernels {
+  using PFn = void (*)(const FftPlaneItem*, int, const LinePlanDev*, const cx<T>*, const SimState*);
+  using XFn = void (*)(const FrXItem*, int, const LinePlanDev*, const cx<T>*, const SimState*);
+  PFn fwd, inv;
+  size_t fwd_lds, inv_lds;
+  int fwd_nt, inv_nt, fwd_parts, inv_parts;
+  XFn x;
+  size_t x_lds;
+  int x_nt;
+};
+template <typename T>
+static WaveKernels<T> wave_kernels() {
+  // FP32: forward planes in blocks of 8 waves, inverse planes of 9 (all 65
+  // spectrum columns in one round), 2 parts each; FP64 (twice the LDS per line):
+  // 4 waves, 2 parts; x lines in blocks of 4 waves
+  if constexpr (sizeof(T) == 4)
+    return {&k_frp_fwd<T, 8, 2>, &k_frp_inv<T, 9, 2>, FrPlaneFwd<T, 8, 2>::BYTES, FrPlaneInv<T, 9, 2>::BYTES,
+            8 * 64, 9 * 64, 2, 2, &k_frx<T, 4>, FrX<T, 4>::BYTES, 4 * 64};
+  else
+    return {&k_frp_fwd<T, 4, 2>, &k_frp_inv<T, 4, 2>, FrPlaneFwd<T, 4, 2>::BYTES, FrPlaneInv<T, 4, 2>::BYTES,
+            4 * 64, 4 * 64, 2, 2, &k_frx<T, 4>, FrX<T, 4>::BYTES, 4 * 64};
+}
+
 // the five stages of psi = irfftn(Bhat rfftn(s)) and their work items
 template <typename T>
 struct FftBatch {
@@ -355,13 +383,22 @@ struct FftBatch {
   int parts = 0, pnt = 256;  // parts per plane (0 = line passes only), threads per block
   // plane items per plane length (one launch per length and direction)
   struct PlaneGroup {
-    PlaneKernels<T> k;
+    // per direction: kernel, LDS bytes, threads per block
+    typename WaveKernels<T>::PFn kf = nullptr, ki = nullptr;
+    size_t lf = 0, li = 0;
+    int nf = 0, ni = 0;
     std::vector<FftPlaneItem> fwd, inv;
     int off = 0;  // first item in d_pitems (forward items, then inverse items)
   };
   std::map<int, PlaneGroup> pgroups;
   FftPlaneItem* d_pitems = nullptr;
   bool has_planes() const { return !pgroups.empty(); }
+  // wave-resident passes (mask, FFT_WAVE_ALL), where lengths are 128
+  int wave = 0;
+  int wave_used = 0;  // the passes that use them in the current items
+  int parts_used = 0;  // parts per plane of the forward plane passes in use
+  std::vector<FrXItem> xitems;
+  FrXItem* d_xitems = nullptr;
 
   static bool paired(int kind) { return kind == FK_R2C || kind == FK_C2R; }
   // complex lines per item: two line buffers within the LDS budget (real-line
@@ -469,22 +506,49 @@ struct FftBatch {
       return;
     }
     const int nx = (int)shape[0], ny = (int)shape[1];
-    PlaneKernels<T> k = (ny == nl) ? plane_kernels<T>(nl, parts, pnt) : PlaneKernels<T>{};
-    k.nt = pnt;
-    if (k.fwd && k.fwd_lds <= (size_t)devinfo().lds && k.inv_lds <= (size_t)devinfo().lds) {
-      // rows and y lines of each plane in the plane passes; the row stages above are dropped
-      stage[0].resize(stage[0].size() - nrows_r2c);
-      stage[4].resize(stage[4].size() - nrows_c2r);
+    const size_t lmax = (size_t)devinfo().lds;
+    const WaveKernels<T> wk = wave_kernels<T>();
+    // per direction: the wave-resident plane pass, else the Stockham plane pass
+    // (parts > 0), else the line passes (rows, then y lines)
+    const bool wave_len = (ny == FR_N && nl == FR_N);
+    PlaneKernels<T> ok{};
+    if (ny == nl && parts > 0) ok = plane_kernels<T>(nl, parts, pnt);
+    const bool wf = (wave & 1) && wave_len && wk.fwd_lds <= lmax;
+    const bool wi = (wave & 4) && wave_len && wk.inv_lds <= lmax;
+    const bool of = !wf && ok.fwd && ok.fwd_lds <= lmax && ok.inv_lds <= lmax;
+    const bool oi = !wi && ok.inv && ok.fwd_lds <= lmax && ok.inv_lds <= lmax;
+    if (wf || of || wi || oi) {
       PlaneGroup& pg = pgroups[nl];
-      pg.k = k;
       const int plan = plans.get(nl);
-      for (int x = 0; x < nx; ++x)
-        for (int p = 0; p < parts; ++p) {
-          pg.fwd.push_back({sim, plan, x, p, s, C, (int)nhp});
-          pg.inv.push_back({sim, plan, x, p, C, psi, (int)nhp});
-        }
-    } else {
-    // y passes: for each x plane, lines kz in [0, nh) with point stride nhp
+      if (wf || of) {
+        // rows and y lines of each plane in the forward plane pass; its row stage above is dropped
+        stage[0].resize(stage[0].size() - nrows_r2c);
+        const auto kf = wf ? wk.fwd : ok.fwd;
+        require(pg.fwd.empty() || pg.kf == kf, "FFT plane passes of one length must share their kernels");
+        pg.kf = kf;
+        pg.lf = wf ? wk.fwd_lds : ok.fwd_lds;
+        pg.nf = wf ? wk.fwd_nt : pnt;
+        const int pf = wf ? wk.fwd_parts : parts;
+        for (int x = 0; x < nx; ++x)
+          for (int p = 0; p < pf; ++p) pg.fwd.push_back({sim, plan, x, p, s, C, (int)nhp});
+        parts_used = pf;
+        if (wf) wave_used |= 1;
+      }
+      if (wi || oi) {
+        stage[4].resize(stage[4].size() - nrows_c2r);
+        const auto ki = wi ? wk.inv : ok.inv;
+        require(pg.inv.empty() || pg.ki == ki, "FFT plane passes of one length must share their kernels");
+        pg.ki = ki;
+        pg.li = wi ? wk.inv_lds : ok.inv_lds;
+        pg.ni = wi ? wk.inv_nt : pnt;
+        const int pi = wi ? wk.inv_parts : parts;
+        for (int x = 0; x < nx; ++x)
+          for (int p = 0; p < pi; ++p) pg.inv.push_back({sim, plan, x, p, C, psi, (int)nhp});
+        if (wi) wave_used |= 4;
+      }
+    }
+    // y lines of the directions without a plane pass: for each x plane, lines kz in
+    // [0, nh) with point stride nhp
     for (int x = 0; x < nx; ++x) {
       FftItem y{};
       y.sim = sim;
@@ -493,11 +557,25 @@ struct FftBatch {
       y.out = C + (long)x * ny * nhp;
       y.ps_in = y.ps_out = nhp;
       y.ls_in = y.ls_out = 1;
-      y.kind = FK_FWD;
-      add_lines(1, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
-      y.kind = FK_INV;
-      add_lines(3, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
+      if (!(wf || of)) {
+        y.kind = FK_FWD;
+        add_lines(1, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
+      }
+      if (!(wi || oi)) {
+        y.kind = FK_INV;
+        add_lines(3, y, ny, nh, sizeof(cx<T>), sizeof(cx<T>), 0, al);
+      }
     }
+    if ((wave & 2) && nx == FR_N && wk.x_lds <= lmax) {
+      // x pass, wave-resident: items of (waves per block) x 8 lines (ky, kz), kz fastest
+      const int plan = plans.get(nx);
+      const int per = wk.x_nt / 8;
+      const long nlines = (long)ny * nh;
+      for (long l0 = 0; l0 < nlines; l0 += per)
+        xitems.push_back({sim, plan, (int)l0, (int)std::min<long>(per, nlines - l0), ny, (int)nh, (int)nhp, C,
+                          bhat});
+      wave_used |= 2;
+      return;
     }
     // x pass with Bhat: for each y, lines kz in [0, nh) with point stride ny*nhp
     for (int y = 0; y < ny; ++y) {
@@ -516,12 +594,21 @@ struct FftBatch {
     for (auto& s : stage) s.clear();
     for (auto& l : lds) l = 0;
     pgroups.clear();
+    xitems.clear();
+    wave_used = 0;
+    parts_used = 0;
   }
   void upload() {
     dfree(d_items);
     d_items = nullptr;
     dfree(d_pitems);
     d_pitems = nullptr;
+    dfree(d_xitems);
+    d_xitems = nullptr;
+    if (!xitems.empty()) {
+      d_xitems = (FrXItem*)dmalloc(xitems.size() * sizeof(FrXItem));
+      HIPCHK(hipMemcpy(d_xitems, xitems.data(), xitems.size() * sizeof(FrXItem), hipMemcpyHostToDevice));
+    }
     if (has_planes()) {
       std::vector<FftPlaneItem> pall;
       for (auto& kv : pgroups) {
@@ -546,18 +633,29 @@ struct FftBatch {
     const DevInfo& di = devinfo();
     for (auto& kv : pgroups) {
       const PlaneGroup& pg = kv.second;
-      const int n = (int)pg.fwd.size();
+      const int nf = (int)pg.fwd.size(), n = inverse ? (int)pg.inv.size() : nf;
+      if (n == 0) continue;
       const unsigned grid = xcd_grid(std::min<long>(n, (long)di.cus * 8));
-      hipLaunchKernelGGL(inverse ? pg.k.inv : pg.k.fwd, dim3(grid), dim3(pg.k.nt),
-                         inverse ? pg.k.inv_lds : pg.k.fwd_lds, stream, d_pitems + pg.off + (inverse ? n : 0), n,
-                         plans.d_plans, plans.d_tw, st);
+      hipLaunchKernelGGL(inverse ? pg.ki : pg.kf, dim3(grid), dim3(inverse ? pg.ni : pg.nf),
+                         inverse ? pg.li : pg.lf, stream, d_pitems + pg.off + (inverse ? nf : 0), n, plans.d_plans,
+                         plans.d_tw, st);
       HIPCHK(hipGetLastError());
     }
+  }
+  void run_x(hipStream_t stream, const SimState* st) {
+    if (xitems.empty()) return;
+    const WaveKernels<T> wk = wave_kernels<T>();
+    const int n = (int)xitems.size();
+    const unsigned grid = xcd_grid(std::min<long>(n, (long)devinfo().cus * 8));
+    hipLaunchKernelGGL(wk.x, dim3(grid), dim3(wk.x_nt), wk.x_lds, stream, d_xitems, n, plans.d_plans, plans.d_tw,
+                       st);
+    HIPCHK(hipGetLastError());
   }
   void run(hipStream_t stream, const SimState* st) {
     const DevInfo& di = devinfo();
     run_planes(stream, st, false);
     for (int i = 0; i < 5; ++i) {
+      if (i == 2) run_x(stream, st);
       const int n = offs[i + 1] - offs[i];
       if (n == 0) continue;
       require(lds[i] <= (size_t)di.lds, "FFT line too long for LDS: " + std::to_string(lds[i]) + " bytes");
@@ -581,6 +679,7 @@ struct FftBatch {
   ~FftBatch() {
     dfree(d_items);
     dfree(d_pitems);
+    dfree(d_xitems);
   }
 };
 
@@ -735,6 +834,7 @@ struct EngineBase {
   virtual void set_tiling(int variant, int cx) = 0;
   virtual void set_fft(int budget_kb, int nt) = 0;
   virtual void set_fft_planes(int parts, int nt) = 0;
+  virtual void set_fft_wave(int mode) = 0;
   virtual int num_variants(int* info) = 0;
   virtual void launch_info(long* out) = 0;
   virtual int num_groups() = 0;
@@ -919,6 +1019,7 @@ struct Engine : EngineBase {
     fft.nt = 256;
     fft.parts = sizeof(T) == 4 ? 2 : 4;
     fft.pnt = 1024;
+    fft.wave = sizeof(T) == 4 ? 1 : FFT_WAVE_ALL;
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
@@ -931,6 +1032,12 @@ struct Engine : EngineBase {
           (void)hipFuncSetAttribute((const void*)k.fwd, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
           (void)hipFuncSetAttribute((const void*)k.inv, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
         }
+    {
+      const WaveKernels<T> k = wave_kernels<T>();
+      (void)hipFuncSetAttribute((const void*)k.fwd, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+      (void)hipFuncSetAttribute((const void*)k.inv, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+      (void)hipFuncSetAttribute((const void*)k.x, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
+    }
   }
   ~Engine() override {
     drop_graphs();
@@ -1275,6 +1382,7 @@ struct Engine : EngineBase {
         g->fft.nt = fft.nt;
         g->fft.parts = fft.parts;
         g->fft.pnt = fft.pnt;
+        g->fft.wave = fft.wave;
         for (int i : g->sims) {
           const Sim& s = sims[i];
           if (s.d.elastic) g->fft.add_sim(i, s.d.dim, s.d.shape, s.s, s.C, s.psi, s.bhat);
@@ -1579,6 +1687,11 @@ struct Engine : EngineBase {
     fft.pnt = nt;
     dirty = true;
   }
+  void set_fft_wave(int mode) override {
+    require(mode >= 0 && mode <= FFT_WAVE_ALL, "FFT wave mode must be a mask in 0.." + std::to_string(FFT_WAVE_ALL));
+    fft.wave = mode;
+    dirty = true;
+  }
   void set_fft(int budget_kb, int nt) override {
     require(budget_kb >= 4 && budget_kb <= 160, "FFT LDS budget must be 4..160 KiB");
     require(nt == 64 || nt == 128 || nt == 256 || nt == 512, "FFT block size must be 64, 128, 256 or 512");
@@ -1712,8 +1825,9 @@ struct Engine : EngineBase {
     out[7] = (long)fft.budget;
     out[8] = fft.nt;
     out[9] = kf_grid;
-    out[10] = fft.has_planes() ? fft.parts : 0;
+    out[10] = fft.parts_used;
     out[11] = fft.pnt;
+    out[12] = fft.wave_used;
   }
 };
 
@@ -2167,6 +2281,9 @@ int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* info) {
 int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts, int nt) {
   return guarded([&] { e->impl->set_fft_planes(parts, nt); });
 }
+int pf5d_engine_set_fft_wave(pf5d_engine* e, int mode) {
+  return guarded([&] { e->impl->set_fft_wave(mode); });
+}
 int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt) {
   return guarded([&] { e->impl->set_fft(budget_kb, nt); });
 }
@@ -2216,7 +2333,7 @@ int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups) {
   return guarded([&] { *ngroups = e->impl->num_groups(); });
 }
 
-int pf5d_engine_launch_info(pf5d_engine* e, long out[12]) {
+int pf5d_engine_launch_info(pf5d_engine* e, long out[13]) {
   return guarded([&] { e->impl->launch_info(out); });
 }
 

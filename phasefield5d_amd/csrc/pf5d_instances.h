@@ -7,6 +7,7 @@
 #pragma once
 #include "pf5d_fft.h"
 #include "pf5d_fftplane.h"
+#include "pf5d_fftreg.h"
 #include "pf5d_fused.h"
 #include "pf5d_split.h"
 
@@ -27,6 +28,23 @@ namespace pf5d {
   PF5D_INST __global__ void k_fftp_inv<T, NT, N, P>(const FftPlaneItem* __restrict__, int,                 \
                                                     const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
                                                     const SimState* __restrict__);
+#define PF5D_FRP_INSTANCE(T, NW, P)                                                                      \
+  PF5D_INST __global__ void k_frp_fwd<T, NW, P>(const FftPlaneItem* __restrict__, int,                  \
+                                                const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
+                                                const SimState* __restrict__);                         \
+  PF5D_INST __global__ void k_frp_inv<T, NW, P>(const FftPlaneItem* __restrict__, int,                  \
+                                                const LinePlanDev* __restrict__, const cx<T>* __restrict__, \
+                                                const SimState* __restrict__);
+#define PF5D_FRX_INSTANCE(T, NW)                                                                           \
+  PF5D_INST __global__ void k_frx<T, NW>(const FrXItem* __restrict__, int, const LinePlanDev* __restrict__, \
+                                         const cx<T>* __restrict__, const SimState* __restrict__);
+// the wave-resident passes of 128-point planes and x lines: (waves per block, parts per plane)
+#define PF5D_FR_INSTANCES         \
+  PF5D_FRP_INSTANCE(float, 8, 2)  \
+  PF5D_FRP_INSTANCE(float, 9, 2)  \
+  PF5D_FRP_INSTANCE(double, 4, 2) \
+  PF5D_FRX_INSTANCE(float, 4)     \
+  PF5D_FRX_INSTANCE(double, 4)
 #define PF5D_STENCIL_INSTANCE(T, BY, BZ, NT, MINW)                                                 \
   PF5D_INST __global__ void k_stencil<T, BY, BZ, NT, MINW>(const SimDev<T>* __restrict__,          \
                                                            SimState* __restrict__, T* __restrict__, \

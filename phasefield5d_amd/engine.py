@@ -104,13 +104,16 @@ class Engine:
     def set_tiling(self, variant, cx):
         _lib.check(_lib.lib().pf5d_engine_set_tiling(self.h, int(variant), int(cx)))
 
-    def set_fft(self, budget_kb, nt, parts=None, plane_nt=256):
+    def set_fft(self, budget_kb, nt, parts=None, plane_nt=256, wave=None):
         """Line-pass batching (LDS KiB per item, threads per block) and, when given,
-        the parts per plane of the 3D plane passes (0 = line passes only) and
-        their block size."""
+        the parts per plane of the 3D plane passes (0 = line passes only), their
+        block size, and which passes run wave-resident where lengths are 128 (mask:
+        1 forward plane pass, 2 x pass, 4 inverse plane pass; pf5d_engine_set_fft_wave)."""
         _lib.check(_lib.lib().pf5d_engine_set_fft(self.h, int(budget_kb), int(nt)))
         if parts is not None:
             _lib.check(_lib.lib().pf5d_engine_set_fft_planes(self.h, int(parts), int(plane_nt)))
+        if wave is not None:
+            _lib.check(_lib.lib().pf5d_engine_set_fft_wave(self.h, int(wave)))
 
     # -- readback ----------------------------------------------------------------
     def state(self, slot):
@@ -187,7 +190,7 @@ class Engine:
         return n.value
 
     def launch_info(self):
-        out = (C.c_long * 12)()
+        out = (C.c_long * 13)()
         _lib.check(_lib.lib().pf5d_engine_launch_info(self.h, out))
         return dict(zip(("k1_items", "k2_items", "fft_items", "k1_grid", "k1_lds", "variant", "cx", "fft_budget",
-                         "fft_nt", "kf_grid", "fft_parts", "fft_plane_nt"), list(out)))
+                         "fft_nt", "kf_grid", "fft_parts", "fft_plane_nt", "fft_wave"), list(out)))

@@ -390,19 +390,26 @@ def test_fft_plane_passes_bitwise(dev, precision):
     pb = problem(3, 128, seed=9)
     pb.X_init = developed_state(pb, amp=0.03)
     out, used = {}, {}
-    for parts in (0, 2, 4, 8):
+    # (parts, wave mask): the Stockham plane passes, then the wave-resident passes
+    # (1 forward planes, 2 x lines, 4 inverse planes), alone and mixed with the others
+    configs = [(0, 0), (2, 0), (4, 0), (8, 0), (0, 1), (0, 2), (0, 4), (0, 7), (4, 2), (4, 5)]
+    for parts, wave in configs:
+        print(f"[fft] {precision} parts {parts} wave {wave}", flush=True)
         e = Engine(precision)
         s = e.add_problem(pb)
-        e.set_fft(16, 128, parts)
+        e.set_fft(16, 128, parts, wave=wave)
         e.step(2)
-        out[parts] = (e.field(s, "X"), e.field(s, "psi"), e.state(s)["dt"])
-        used[parts] = e.launch_info()["fft_parts"]
+        out[parts, wave] = (e.field(s, "X"), e.field(s, "psi"), e.state(s)["dt"])
+        info = e.launch_info()
+        used[parts, wave] = (info["fft_parts"], info["fft_wave"])
         e.close()
-    assert used[0] == 0 and used[4] == 4 and used[8] == 8
-    assert used[2] == (2 if precision == "fp32" else 0)
-    for parts in (2, 4, 8):
-        for a, b in zip(out[0], out[parts]):
-            assert np.array_equal(a, b), f"parts {parts}"
+    assert used[0, 0] == (0, 0) and used[4, 0] == (4, 0) and used[8, 0] == (8, 0)
+    assert used[2, 0] == ((2, 0) if precision == "fp32" else (0, 0))
+    assert used[0, 1] == (2, 1) and used[0, 2] == (0, 2) and used[0, 4] == (0, 4) and used[0, 7] == (2, 7)
+    assert used[4, 2] == (4, 2) and used[4, 5] == (2, 5)
+    for cfg in configs[1:]:
+        for a, b in zip(out[0, 0], out[cfg]):
+            assert np.array_equal(a, b), f"parts, wave {cfg}"
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -415,16 +422,18 @@ def test_fft_plane_passes_mixed_lengths_bitwise(dev, precision):
     for p in pbs:
         p.X_init = developed_state(p, amp=0.03)
     out = {}
-    for parts in (0, 4):
+    for parts, wave in ((0, 0), (4, 0), (4, 7)):
         e = Engine(precision)
         slots = [e.add_problem(p) for p in pbs]
-        e.set_fft(16, 128, parts, 512)
+        e.set_fft(16, 128, parts, 512, wave=wave)
         e.step(2)
-        out[parts] = [(e.field(s, "X"), e.field(s, "psi")) for s in slots]
-        assert e.launch_info()["fft_parts"] == parts
+        out[parts, wave] = [(e.field(s, "X"), e.field(s, "psi")) for s in slots]
+        info = e.launch_info()
+        assert info["fft_wave"] == wave and (info["fft_parts"] > 0) == (parts > 0)
         e.close()
-    for (Xa, pa), (Xb, pb_) in zip(out[0], out[4]):
-        assert np.array_equal(Xa, Xb) and np.array_equal(pa, pb_)
+    for cfg in ((4, 0), (4, 7)):
+        for (Xa, pa), (Xb, pb_) in zip(out[0, 0], out[cfg]):
+            assert np.array_equal(Xa, Xb) and np.array_equal(pa, pb_), cfg
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
