@@ -135,6 +135,7 @@ struct pf5d_table {
   void* data;
   void* cubes;  // cube table of the linear region (null when num < 5)
   int cube_s;
+  bool complete;  // every node of the simplex holds a row (no NaN from a missing one)
 };
 
 template <typename T>
@@ -178,6 +179,16 @@ static void upload_table(pf5d_table* t, const double* grid, int nprops) {
   std::vector<T> h((size_t)nn * NP, T(0));
   for (long i = 0; i < nn; ++i)
     for (int p = 0; p < nprops; ++p) h[i * NP + p] = (T)grid[i * nprops + p];
+  t->complete = true;
+  const int n = t->num;
+  for (int a = 0; a < n && t->complete; ++a)
+    for (int b = 0; a + b < n && t->complete; ++b)
+      for (int c = 0; a + b + c < n && t->complete; ++c)
+        for (int d = 0; a + b + c + d < n; ++d)
+          if (std::isnan(grid[((((long)a * n + b) * n + c) * n + d) * nprops])) {
+            t->complete = false;
+            break;
+          }
   t->data = dmalloc(h.size() * sizeof(T));
   HIPCHK(hipMemcpy(t->data, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   // the cube table: lower corners with index sum <= num - 5 hold every point
@@ -864,6 +875,8 @@ struct K1Variant {
   void (*fmarch)(const SimDev<T>*, SimState*, T*, const K1Item*, int, const double*, const int*, int, const int*,
                  unsigned*) = nullptr;
   int fminw = 0;  // marching pass: min waves per SIMD it was compiled for
+  // potentials pass with the nearest-row search for tables with missing rows
+  void (*fpot_wide)(const SimDev<T>*, const SimState*, const K1Item*, int) = nullptr;
   bool split() const { return by == 0; }
 };
 
@@ -876,7 +889,9 @@ static K1Variant<T> make_variant() {
 // z neighbours from adjacent lanes); BZ: row length of a tile
 template <typename T, int NT, int MINW, int LOADS, int BZ = 64>
 static K1Variant<T> make_split_variant() {
-  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, &k_fluxdiv<T, NT, LOADS>, 0, NT};
+  K1Variant<T> v{0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW, false>, &k_fluxdiv<T, NT, LOADS>, 0, NT};
+  v.fpot_wide = &k_potentials<T, NT, MINW, true>;
+  return v;
 }
 // two-pass form with the marching flux pass (FNT threads, CPL cells per lane,
 // FMINW waves per SIMD at least)
@@ -884,7 +899,10 @@ static K1Variant<T> make_split_variant() {
 // developed field's cells close to its node box's anchor)
 template <typename T, int NT, int MINW, int FNT, int CPL, int FMINW = 1, int BZ = 16>
 static K1Variant<T> make_march_variant() {
-  return {0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL, FMINW>, FMINW};
+  K1Variant<T> v{0,   BZ,  NT, 0, nullptr, &k_potentials<T, NT, MINW, false>, nullptr, CPL, FNT,
+                 &k_fluxmarch<T, FNT, CPL, FMINW>, FMINW};
+  v.fpot_wide = &k_potentials<T, NT, MINW, true>;
+  return v;
 }
 
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
@@ -1015,6 +1033,7 @@ struct Engine : EngineBase {
     for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
       if (!variants[i].split() && variants[i].lds <= (size_t)devinfo().lds) variant = i;
     require(variant >= 0, "no stencil tiling fits this device's LDS");
+    default_variant = variant;
     fft.budget = 32 * 1024;
     fft.nt = 256;
     fft.parts = sizeof(T) == 4 ? 2 : 4;
@@ -1266,9 +1285,21 @@ struct Engine : EngineBase {
     v.tol_hard = d.tol_hard;
     return v;
   }
+  // a simulation's table misses rows of the simplex: its nearest-row search must
+  // skip them (the WIDE potentials pass; the single-pass stencil lacks it)
+  bool wide = false;
+  int default_variant = 0;
+  void (*pot(const K1Variant<T>& V) const)(const SimDev<T>*, const SimState*, const K1Item*, int) {
+    return wide ? V.fpot_wide : V.fpot;
+  }
   void rebuild() {
     HIPCHK(hipStreamSynchronize(stream));  // buffers below may still be in use
     drop_graphs();
+    wide = false;
+    for (const auto& s : sims)
+      if (s.used && s.tab && !s.tab->complete) wide = true;
+    // the single-pass tilings search complete tables only: fall back to the default two-pass form
+    if (wide && !variants[variant].split()) variant = default_variant;
     const K1Variant<T>& V = variants[variant];
     std::vector<SimDev<T>> devs(cap);
     std::vector<K1Item> k1, kf;
@@ -1424,14 +1455,14 @@ struct Engine : EngineBase {
     int occ = 1;
     if (V.split()) {
       auto grid = [&](int o) { return xcd_grid(std::min(nk1, devinfo().cus * std::max(1, o))); };
-      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fpot, V.nt, 0));
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)pot(V), V.nt, 0));
       k1_grid = grid(occ);
       occ = 1;
       const void* ff = V.fcpl > 0 ? (const void*)V.fmarch : (const void*)V.fflux;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ff, V.fnt, 0));
       kf_grid = V.fcpl > 0 ? xcd_grid(std::min(nkf, devinfo().cus * std::max(1, occ))) : grid(occ);
       int occ_pot = 1;
-      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_pot, (const void*)V.fpot, V.nt, 0));
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_pot, (const void*)pot(V), V.nt, 0));
       for (auto& g : groups) {
         g->k1_grid = xcd_grid(std::max(1, std::min(g->nk1, devinfo().cus * std::max(1, occ_pot))));
         g->kf_grid = xcd_grid(std::max(1, std::min(g->nkf, devinfo().cus * std::max(1, occ))));
@@ -1496,7 +1527,7 @@ struct Engine : EngineBase {
     if (!nk1) return false;
     const K1Variant<T>& V = variants[variant];
     if (V.split()) {
-      hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
+      hipLaunchKernelGGL(pot(V), dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
       HIPCHK(hipGetLastError());
       const bool did = launch_flux(V, with_dt);
       HIPCHK(hipGetLastError());
@@ -1518,7 +1549,7 @@ struct Engine : EngineBase {
     const K1Variant<T>& V = variants[variant];
     g.fft.run(stream, d_state);
     if (g.nk1) {
-      hipLaunchKernelGGL(V.fpot, dim3(g.k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1 + g.k1_off, g.nk1);
+      hipLaunchKernelGGL(pot(V), dim3(g.k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1 + g.k1_off, g.nk1);
       HIPCHK(hipGetLastError());
     }
     const int ns = (int)g.sims.size();
@@ -1727,7 +1758,7 @@ struct Engine : EngineBase {
       else if (which == 2) launch_dt(0);
       else if (which == 3) launch_k2(1);
       else if (which == 4)
-        hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
+        hipLaunchKernelGGL(pot(V), dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
       else
         launch_flux(V);
     };
@@ -1779,7 +1810,7 @@ struct Engine : EngineBase {
         HIPCHK(hipEventRecord(e[0], stream));
         fft.run(stream, d_state);
         HIPCHK(hipEventRecord(e[1], stream));
-        hipLaunchKernelGGL(V.fpot, dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
+        hipLaunchKernelGGL(pot(V), dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(e[2], stream));
         fused = launch_flux(V, !separate_dt);

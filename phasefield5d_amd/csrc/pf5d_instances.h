@@ -49,9 +49,13 @@ namespace pf5d {
   PF5D_INST __global__ void k_stencil<T, BY, BZ, NT, MINW>(const SimDev<T>* __restrict__,          \
                                                            SimState* __restrict__, T* __restrict__, \
                                                            const K1Item* __restrict__, int);
-#define PF5D_POTENTIALS_INSTANCE(T, NT, MINW)                                                              \
-  PF5D_INST __global__ void k_potentials<T, NT, MINW>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
-                                                      const K1Item* __restrict__, int);
+#define PF5D_POTENTIALS_INSTANCE(T, NT, MINW)                                                                   \
+  PF5D_INST __global__ void k_potentials<T, NT, MINW, false>(const SimDev<T>* __restrict__,                       \
+                                                             const SimState* __restrict__, const K1Item* __restrict__, \
+                                                             int);                                               \
+  PF5D_INST __global__ void k_potentials<T, NT, MINW, true>(const SimDev<T>* __restrict__,                        \
+                                                            const SimState* __restrict__, const K1Item* __restrict__,  \
+                                                            int);
 #define PF5D_FLUXM_INSTANCE(T, NT, CPL, MINW)                                                              \
   PF5D_INST __global__ void k_fluxmarch<T, NT, CPL, MINW>(const SimDev<T>* __restrict__, SimState* __restrict__, \
                                                     T* __restrict__, const K1Item* __restrict__, int,       \

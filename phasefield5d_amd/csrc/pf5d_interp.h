@@ -41,7 +41,7 @@ struct TableView {
 };
 
 // nearest table row for points past the linear region (defined below)
-template <typename T>
+template <typename T, bool WIDE>
 PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]);
 
 // Cube table: for every hypercube of the linear region (lower corner i with
@@ -286,7 +286,7 @@ PF5D_DEV void interp9_staged(const TableView<T>& tv, const T* box, const NodeBox
   constexpr int NP = TablePad<T>::value;
   const T s = ((X[0] + X[1]) + X[2]) + X[3];
   if (!(s <= tv.thresh)) {
-    interp_nearest<T>(tv, X, P);
+    interp_nearest<T, false>(tv, X, P);
     return;
   }
   const int num = tv.num;
@@ -330,12 +330,12 @@ struct BoxLayout {
   static constexpr int SLOTS = BOX4 * S0;
   static_assert(NCOL % 2 == 1, "the slot count per node must be odd");
 };
-template <typename T>
+template <typename T, bool WIDE>
 PF5D_DEV void interp9_box4(const TableView<T>& tv, const T* box, const int lo[4], const T X[4], T P[9]) {
   constexpr int NP = TablePad<T>::value;
   const T s = ((X[0] + X[1]) + X[2]) + X[3];
   if (!(s <= tv.thresh)) {
-    interp_nearest<T>(tv, X, P);
+    interp_nearest<T, WIDE>(tv, X, P);
     return;
   }
   const int num = tv.num;
@@ -440,36 +440,43 @@ PF5D_DEV NodeBox stage_node_box(const TableView<T>& tab, int v[8], T* box, int* 
 // leaves NaN where a row is absent, interpolation.py:30-34).
 //
 // With u = X/h and f = floor(u) per axis, the boxes B_R = [f - R, f + 1 + R]^4
-// are scanned for R = 0, 1, 2, 4, 8, ...  A node outside B_R is at least (R + 1) h
-// away on some axis, so once the best squared distance found is at most
-// ((R + 1) h)^2 no node outside can be nearer and the search stops.  B_0 holds
-// the per-axis nearest node, within h of the point: when it is a row (the usual
-// case) the first box settles it.  Nodes are visited in lexicographic order and
-// a candidate replaces the best only when strictly nearer, so equidistant rows
-// resolve to the first in that order at every R.  Missing rows are skipped by
-// reading a candidate's first value only when it would become the best.
-template <typename T>
-PF5D_DEV void nearest_scan_box(const TableView<T>& tv, const double X[4], const int lo[4], const int hi[4],
-                               double& best, long& bidx) {
+// are scanned, R = 0 first.  A node outside B_R is at least (R + 1) h away on
+// some axis, so once the best squared distance found is at most ((R + 1) h)^2
+// no node outside can be nearer.  B_0 holds the per-axis nearest node, within
+// h of the point: when it is a row (the usual case) the first box settles it;
+// else R = 1, 2, 4, ... until the bound holds.  Nodes are visited in
+// lexicographic order and a candidate replaces the best only when strictly
+// nearer, so equidistant rows resolve to the first in that order at every R.
+// Missing rows are skipped by reading a candidate's first value only when it
+// would become the best.
+template <typename T, bool WIDE>
+PF5D_DEV void nearest_scan(const TableView<T>& tv, const T X[4], const int lo[4], int a0, int a1, double& best,
+                           long& bidx) {
   constexpr int NP = TablePad<T>::value;
   const int num = tv.num;
   const double h = tv.step;
 #pragma unroll 1
-  for (int n0 = lo[0]; n0 <= hi[0]; ++n0) {
-    const double d0 = X[0] - n0 * h;
+  for (int a = a0; a < a1; ++a) {
+    const int n0 = lo[0] + a;
+    if (n0 < 0 || n0 >= num) continue;
 #pragma unroll 1
-    for (int n1 = lo[1]; n1 <= hi[1] && n0 + n1 <= num - 1; ++n1) {
-      const double d1 = X[1] - n1 * h;
+    for (int b = a0; b < a1; ++b) {
+      const int n1 = lo[1] + b;
+      if (n1 < 0 || n1 >= num || n0 + n1 > num - 1) continue;
 #pragma unroll 1
-      for (int n2 = lo[2]; n2 <= hi[2] && n0 + n1 + n2 <= num - 1; ++n2) {
-        const double d2 = X[2] - n2 * h;
+      for (int c = a0; c < a1; ++c) {
+        const int n2 = lo[2] + c;
+        if (n2 < 0 || n2 >= num || n0 + n1 + n2 > num - 1) continue;
 #pragma unroll 1
-        for (int n3 = lo[3]; n3 <= hi[3] && n0 + n1 + n2 + n3 <= num - 1; ++n3) {
-          const double d3 = X[3] - n3 * h;
+        for (int e = a0; e < a1; ++e) {
+          const int n3 = lo[3] + e;
+          if (n3 < 0 || n3 >= num || n0 + n1 + n2 + n3 > num - 1) continue;
+          const double d0 = (double)X[0] - n0 * h, d1 = (double)X[1] - n1 * h;
+          const double d2 = (double)X[2] - n2 * h, d3 = (double)X[3] - n3 * h;
           const double dd = d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3;
           if (dd < best) {
             const long row = (((long)n0 * num + n1) * num + n2) * num + n3;
-            if (!__builtin_isnan((double)gld<T>(tv.data + row * NP))) {
+            if (!WIDE || !__builtin_isnan((double)gld<T>(tv.data + row * NP))) {
               best = dd;
               bidx = row;
             }
@@ -480,39 +487,57 @@ PF5D_DEV void nearest_scan_box(const TableView<T>& tv, const double X[4], const 
   }
 }
 
-template <typename T>
+// WIDE = false, for the step kernels of engines whose tables have every row of
+// the simplex: the box B_0 (16 nodes), then B_1 (4^4) when the per-axis nearest
+// node lies outside the simplex, without the missing-row test -- exact for
+// points within two grid steps of the simplex, which is where clamp_simplex
+// keeps the step's fields (a point further out gets the nearest row of B_1, or
+// NaN when B_1 holds none).  The widening search costs the potentials pass
+// registers on its main path even where it never runs (7 us of 28 at 128^3
+// FP32), so engines use it only when a table misses rows.
+template <typename T, bool WIDE>
 PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
   constexpr int NP = TablePad<T>::value;
   const int num = tv.num;
   const double h = tv.step;
-  double x[4];
-  int f[4];
-  bool finite = true;
+  int lo[4];
+  bool finite = true, near_in = true;
+  int near_sum = 0;
 #pragma unroll
   for (int d = 0; d < 4; ++d) {
-    x[d] = (double)X[d];
-    finite = finite && __builtin_isfinite(x[d]);
-    const double u = x[d] / h;
-    f[d] = (u <= 0.0) ? 0 : (u >= num - 1) ? num - 1 : (int)u;
+    const double u = (double)X[d] / h;
+    if constexpr (WIDE) finite = finite && __builtin_isfinite(u);  // (NaN distances find no row anyway)
+    // (this form of the clamp keeps the potentials pass's registers at their
+    // best allocation; a NaN or huge u converts to 0 or the largest int on gfx950)
+    int f = (u <= 0.0) ? 0 : (int)u;
+    if (f > num - 1) f = num - 1;
+    if constexpr (WIDE) f = (u == u) ? f : 0;
+    lo[d] = f - 1;
+    // per-axis nearest of the candidates f, f + 1 (ties to f, as the scan keeps the first)
+    const double df = (double)X[d] - f * h, dg = (double)X[d] - (f + 1) * h;
+    const int r = (f + 1 < num && dg * dg < df * df) ? f + 1 : f;
+    near_sum += r;
   }
+  near_in = near_sum <= num - 1;
   double best = 1e300;
   long bidx = -1;
   if (finite) {
+    nearest_scan<T, WIDE>(tv, X, lo, 1, 3, best, bidx);
+    if constexpr (WIDE) {
+      int R = 0;
 #pragma unroll 1
-    for (int R = 0;; R = (R == 0) ? 1 : 2 * R) {
-      int lo[4], hi[4];
-      bool whole = true;
-#pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        lo[d] = max(0, f[d] - R);
-        hi[d] = min(num - 1, f[d] + 1 + R);
-        whole = whole && lo[d] == 0 && hi[d] == num - 1;
+      while (!(bidx >= 0 && best <= (double)((R + 1) * (R + 1)) * (h * h)) && R <= 2 * num) {
+        R = (R == 0) ? 1 : 2 * R;
+        best = 1e300;
+        bidx = -1;
+        nearest_scan<T, WIDE>(tv, X, lo, 1 - R, 3 + R, best, bidx);
       }
-      best = 1e300;
-      bidx = -1;
-      nearest_scan_box<T>(tv, x, lo, hi, best, bidx);
-      const double reach = (R + 1) * h;
-      if ((bidx >= 0 && best <= reach * reach) || whole) break;
+    } else {
+      if (!near_in) {
+        best = 1e300;
+        bidx = -1;
+        nearest_scan<T, WIDE>(tv, X, lo, 0, 4, best, bidx);
+      }
     }
   }
   if (bidx < 0) {
@@ -533,7 +558,7 @@ PF5D_DEV void interp9(const TableView<T>& tv, const T X[4], T P[9]) {
   if (s <= tv.thresh)
     interp_linear<T>(tv, X, P);
   else
-    interp_nearest<T>(tv, X, P);
+    interp_nearest<T, true>(tv, X, P);
 }
 
 }  // namespace pf5d

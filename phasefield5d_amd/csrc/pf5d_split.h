@@ -42,8 +42,9 @@ PF5D_DEV bool tile_cell(const SimDev<T>& sd, const K1Item& item, int tid, float 
   return j < item.ty;
 }
 
-// MINW: minimum waves per SIMD the register allocation must allow
-template <typename T, int NT, int MINW>
+// MINW: minimum waves per SIMD the register allocation must allow; WIDE: the
+// nearest-row search for tables with missing rows (interp_nearest)
+template <typename T, int NT, int MINW, bool WIDE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st, const K1Item* __restrict__ items,
              int nitems) {
@@ -118,7 +119,7 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
       V4<T> xp{};
       if (carry && hx) xp = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx)));
       T P[9];
-      interp9_box4<T>(sd.tab, BOX, lo, xc.v, P);
+      interp9_box4<T, WIDE>(sd.tab, BOX, lo, xc.v, P);
       // neighbour sums in the reference order: x, then y, then z (each up + down)
       T acc[4];
       bool first = true;

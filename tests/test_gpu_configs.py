@@ -56,14 +56,22 @@ def _rocfft_psi64(pb, X):
     return torch.fft.irfftn(k, s=tuple(s.shape), dim=dims).cpu().numpy()
 
 
+def _exact_psi(pb, X):
+    """psi with the transform pair in extended precision (scipy.fft on long doubles,
+    about 1e-18 relative), rounded to FP64: the elastic term without FFT rounding."""
+    import scipy.fft as sfft
+    s = O.misfit_scalar(X, pb.X0, pb.l_vec).astype(np.longdouble)
+    k = sfft.rfftn(s) * pb.Bhat.astype(np.longdouble)
+    return sfft.irfftn(k, s=s.shape).astype(np.float64)
+
+
 def test_config_2d256_plain_fp64_per_step():
     """Bar: 1e-12 relative per step.  Every term of the step but the FFT pair is
     bit-exact, so what is left is FFT rounding, amplified as the field develops (the
     potential's terms nearly cancel, dX/dt shrinks).  Where the 1e-12 bar is exceeded,
-    the step must still agree with the oracle as closely as a second FFT library does
-    (within twice the difference the oracle's own step shows when its psi comes from
-    rocFFT, torch.fft in FP64, instead of scipy.fft), and agree to 1e-12 with that
-    rocFFT-psi oracle step."""
+    the FFT's own precision sets the bar: against the step whose psi comes from an
+    extended-precision transform, the device must be no further off than twice the
+    oracle's own scipy.fft step is (rocFFT's, torch.fft in FP64, is reported beside)."""
     from phasefield5d_amd.plain import PlainStepper
     from phasefield5d_amd.problem import build_problem
     from phasefield5d_amd.workloads import workload_config
@@ -88,12 +96,13 @@ def test_config_2d256_plain_fp64_per_step():
             worst = [max(worst[0], e), max(worst[1], ed)]
             assert ed < TOL64, (n, e, ed)
             if e >= TOL64:
+                dXt = _rate_with_psi(pb, Xn, _exact_psi(pb, Xn))
                 dXr = _rate_with_psi(pb, Xn, _rocfft_psi64(pb, Xn))
-                floor, e_roc = rel_inf(dXr, dXo), rel_inf(dXd, dXr)
-                over.append((n, e, floor, e_roc))
-                print(f"[config 2D 256^2 plain] step {n}: dX/dt rel {e:.3e}; rocFFT-psi oracle step vs "
-                      f"scipy-psi oracle step {floor:.3e}; device vs rocFFT-psi oracle step {e_roc:.3e}", flush=True)
-                assert e < 2 * floor and e_roc < TOL64, (n, e, floor, e_roc)
+                e_dev, e_sc, e_roc = rel_inf(dXd, dXt), rel_inf(dXo, dXt), rel_inf(dXr, dXt)
+                over.append((n, e, e_dev, e_sc, e_roc))
+                print(f"[config 2D 256^2 plain] step {n}: device vs oracle {e:.3e}; against the extended-precision "
+                      f"step: device {e_dev:.3e}, oracle (scipy.fft) {e_sc:.3e}, rocFFT {e_roc:.3e}", flush=True)
+                assert e_dev <= max(TOL64, 2 * e_sc), (n, e, e_dev, e_sc)
     print(f"[config 2D 256^2 plain] {len(checks)} steps checked of 20000: max dX/dt rel {worst[0]:.2e}, "
           f"dt rel {worst[1]:.2e}, {len(over)} checks above 1e-12 (FFT-library level), t = {ps.t:.4e} s",
           flush=True)

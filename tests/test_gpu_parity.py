@@ -102,6 +102,45 @@ def test_interpolator_table_with_missing_rows(dev, prec):
         assert np.array_equal(out[fb], values[nn].astype(np.float32).astype(np.float64))
 
 
+def test_engine_step_table_with_missing_rows(dev):
+    """The fused FP64 step (elasticity off: bit-exact) on a table with rows missing
+    where a patch of cells past the linear region finds its nearest rows: the engine
+    takes the potentials pass whose nearest-row search skips missing rows."""
+    from phasefield5d_amd.engine import Engine
+    from phasefield5d_amd.problem import build_problem
+    pb0 = problem(2, 32, elastic=False)
+    h, num = pb0.cfg.resolution, pb0.data_grid.shape[0]
+    X = developed_state(pb0, amp=0.04, seed=3)
+    rng = np.random.default_rng(7)
+    X[10:16, 20:26] = np.array([0.352, 0.298, 0.201, 0.118]) + rng.uniform(-0.01, 0.01, size=(6, 6, 4))
+    Xf = X.reshape(-1, 4)
+    lin = (((Xf[:, 0] + Xf[:, 1]) + Xf[:, 2]) + Xf[:, 3]) <= O.linear_threshold(h)
+    key = lambda a: ((a[..., 0] * num + a[..., 1]) * num + a[..., 2]) * num + a[..., 3]
+    rk = key(np.rint(pb0.rows[:, :4] / h).astype(int))
+    used = set()  # corners the linear cells read: kept, so no linear cell turns NaN
+    for x in Xf[lin]:
+        i = np.clip(np.floor(x / h).astype(int), 0, num - 2)
+        for b in range(16):
+            used.add(int(key(i + np.array([(b >> (3 - d)) & 1 for d in range(4)]))))
+    _, nn = pb0.tree.query(Xf[~lin])
+    gone = set(int(k) for k in rk[nn]) - used
+    pb = build_problem(pb0.cfg, table=pb0.rows[~np.isin(rk, list(gone))], X_init=X)
+    _, nn2 = pb.tree.query(Xf[~lin])
+    moved = (np.abs(pb0.calphad_values[nn] - pb.calphad_values[nn2]).max(axis=1) > 0).sum()
+    d2, _ = pb.tree.query(Xf[~lin], k=2)
+    print(f"[holes] engine: {len(gone)} rows removed, {moved} of {(~lin).sum()} fallback cells move", flush=True)
+    assert moved > 0 and np.all(d2[:, 1] - d2[:, 0] > 1e-9 * h)
+    Xn, dX, dt_new, _, xmax = oracle_sim(pb).step(X, pb.dt0)
+    assert not np.isnan(dX).any()
+    eng = Engine("fp64")
+    slot = eng.add_problem(pb)
+    eng.step(1)
+    st = eng.state(slot)
+    assert np.array_equal(eng.field(slot, "dXdt"), dX)
+    assert st["dt"] == dt_new and np.array_equal(eng.field(slot, "X"), Xn)
+    eng.close()
+
+
 @pytest.mark.parametrize("dim,n", [(1, 160), (2, 48), (3, 24)])
 def test_laplacian_and_divergence_bitwise(dev, dim, n):
     from phasefield5d_amd.solver.fluxes import get_divergence_from_mu_function
