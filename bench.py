@@ -16,15 +16,26 @@ Reported:
                groups that fit the last-level cache), and its ratio to fp64_plain
   fp32_developed_field  the FP32 step on a decomposed-looking field (smooth
                modes of amplitude 0.05 per component), as later in a run
-  roofline     the dominant kernel of the FP32 step: with the two-pass stencil
-               (the autotuner's usual pick) the longer of its two passes by
-               HIP-event duration -- the potentials pass, 56 B/cell (X 16 + psi
-               4 read; mu 16 + Mob 20 written), or the flux pass, 68 B/cell
-               (X 16 + mu 16 + Mob 20 read; dX/dt 16 written) -- with the other
-               pass's figures in roofline_other; with the single-pass stencil
-               K1, 36 B/cell (X 16 + psi 4 read, dX/dt 16 written); achieved =
-               algorithmic bytes / the kernel's HIP-event duration
-  cpu_baseline C/OpenMP FP64 restatement of the reference loop + scipy.fft
+  roofline     the fused stencil stage K1 (interpolation + kappa lap + mu + face
+               fluxes + divergence + block max) against its compulsory traffic,
+               36 B/cell in FP32 (X 16 + psi 4 read, dX/dt 16 written): achieved =
+               36 B x cells / the stage's duration inside the step (HIP events on
+               the engine stream; with the two-pass form, the potentials pass plus
+               the flux pass)
+  roofline_step, roofline_fft  the whole step at 96 B/cell (24 words) and the FFT
+               pair at 8 B/cell (s read, psi written), the same way
+  roofline_passes  each pass of the two-pass stencil against its own bytes,
+               which count the mu / Mob intermediates: 56 B/cell potentials (X,
+               psi in; mu, Mob out), 68 B/cell flux (X, mu, Mob in; dX/dt out)
+  valu         VALU instructions per launch of each kernel (rocprofv3
+               SQ_INSTS_VALU, profiles/step_3d128_fp32_counters.json) over the
+               launch's duration, as a fraction of the issue peak (one wave64
+               instruction per CU per cycle at 2.4 GHz, 256 CUs)
+  rocfft_ms    the same transform pair (rfftn, x Bhat, irfftn of the 128^3 misfit)
+               through torch.fft (hipFFT / rocFFT) on the device, beside the
+               hand-written passes' in-step time
+  cpu_baseline C/OpenMP FP64 restatement of the reference loop + scipy.fft, on all
+               host cores; cpu_baseline_1thread the same on one core
 """
 import argparse
 import dataclasses
@@ -63,8 +74,31 @@ def cpu_baseline(pb, seconds=15.0, threads=None):
         if el >= seconds or n >= 200:
             break
     return {"value": pb.ncell * n / el, "unit": "cell-updates/s", "cores": cp.threads(), "kind": "port",
-            "sample": f"{n} steps of the 3D 128^3 FP64 step (C/OpenMP restatement, scipy.fft 1 worker) "
-                      f"after 1 warm-up step, {el:.1f} s"}
+            "sample": f"{n} steps of the 3D 128^3 FP64 step (C/OpenMP restatement on {cp.threads()} threads, "
+                      f"scipy.fft 1 worker) after 1 warm-up step, {el:.1f} s"}
+
+
+def rocfft_ms(pb, precision, reps=20):
+    """The step's transform pair, psi = irfftn(Bhat rfftn(s)), through torch.fft
+    (hipFFT / rocFFT) on the device: average ms per pair over ``reps`` (CUDA events)."""
+    import torch
+    dt = torch.float32 if precision == "fp32" else torch.float64
+    s = ((pb.X_init - pb.X0) * pb.l_vec).sum(axis=-1)
+    sd = torch.from_numpy(s).to(device="cuda", dtype=dt)
+    bd = torch.from_numpy(pb.Bhat).to(device="cuda", dtype=dt)
+    dims = tuple(range(s.ndim))
+
+    def pair():
+        return torch.fft.irfftn(torch.fft.rfftn(sd, dim=dims) * bd, s=tuple(s.shape), dim=dims)
+    for _ in range(3):
+        pair()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        pair()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / reps
 
 
 def batch_leg(steps, warmup, log_fn):
@@ -91,15 +125,32 @@ def batch_leg(steps, warmup, log_fn):
     cells = sum(pb.ncell for pb in pbs)
     sts = eng.all_states()
     active = sum(1 for s in sts.values() if s["status"] & 1)
+    # FP64 audit of a sample: the first 2D and the first and largest 3D simulation,
+    # each run alone on an FP64 engine for the same steps
+    slots = sorted(eng.slots)
+    pick = [next(i for i, p in enumerate(pbs) if len(p.shape) == 2),
+            next(i for i, p in enumerate(pbs) if len(p.shape) == 3),
+            max(range(len(pbs)), key=lambda i: pbs[i].ncell)]
+    audit = []
+    for i in dict.fromkeys(pick):
+        X32 = eng.field(slots[i], "X")
+        e64 = Engine("fp64")
+        s64 = e64.add_problem(pbs[i])
+        e64.step(warmup + steps)
+        d = np.abs(X32 - e64.field(s64, "X"))
+        audit.append({"shape": list(pbs[i].shape), "steps": warmup + steps, "max": float(d.max()),
+                      "rms": float(np.sqrt((d ** 2).mean()))})
+        e64.close()
     eng.close()
     return {"sims": len(pbs), "cells": cells, "steps": steps, "cell_updates_per_s": cells * steps / el,
-            "ms_per_step": 1e3 * el / steps, "active_after": active, "tiling": tuned}
+            "ms_per_step": 1e3 * el / steps, "active_after": active, "tiling": tuned,
+            "audit_vs_fp64": audit}
 
 
 def serve_leg(duration, rate, max_steps, log_fn):
     """Poisson arrivals of mixed 1D/2D/3D requests into one FP32 continuous-batching server."""
     from phasefield5d_amd.serve import Server, audit, poisson_requests, summarize
-    reqs = poisson_requests(rate, duration, seed=0, max_steps=max_steps, audit_frac=0.05)
+    reqs = poisson_requests(rate, duration, seed=0, max_steps=max_steps, audit_frac=0.1)
     srv = Server("fp32", poll_steps=25, setup_workers=6)
     done = srv.run(reqs, max_wall=duration * 6 + 60)
     srv.close()
@@ -108,7 +159,12 @@ def serve_leg(duration, rate, max_steps, log_fn):
             audit(r)
     out = summarize(done, srv.wall)
     out.update({"arrival_rate_per_s": rate, "arrival_window_s": duration, "submitted": len(reqs),
-                "max_steps_per_request": max_steps, "gpu_busy_s": srv.busy})
+                "max_steps_per_request": max_steps, "gpu_busy_s": srv.busy,
+                "audited_fraction": sum(1 for r in done if r.audit) / max(1, len(done)),
+                "variant": f"reduced serving workload: Poisson arrivals at {rate}/s for {duration:g} s (the "
+                           f"workload names 10 min), each request ending at the earlier of 50 CTLD and "
+                           f"{max_steps} CFL-limited steps (50 CTLD is 1e5-1e6 steps at the reference's dx); "
+                           f"audited requests start from the host's seeded draw and are re-run alone in FP64"})
     return out
 
 
@@ -258,7 +314,7 @@ def main():
 
     mass0 = pb.X_init.reshape(-1, 4).sum(axis=0)
     drift32 = np.abs(st32["mass"] - mass0) / mass0
-    v64 = k1_64 = el64 = vplain = None
+    v64 = k1_64 = el64 = vplain = fft_64 = None
     diff = np.zeros(1)
     long_err = None
     if not args.fp32_only:
@@ -267,6 +323,7 @@ def main():
         e64, s64, el64 = run_engine("fp64", args.steps, args.warmup)
         v64 = ncell * args.steps / el64
         k1_64 = e64.time_kernel("k1", reps=10)
+        fft_64 = e64.time_kernel("fft", reps=10)
         X32, X64 = e32.field(s32, "X"), e64.field(s64, "X")
         diff = np.abs(X32 - X64)
         log(f"fp64 fused: {v64:.4g} cell-updates/s; max|dX| {diff.max():.3e} rms {np.sqrt((diff**2).mean()):.3e}")
@@ -316,14 +373,21 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    def roofline(kname, kfile, bpc, kms, alone_ms=None):
-        """achieved = algorithmic bytes / the launch's average duration inside the
-        step (kms); alone_ms: the same kernel launched repeatedly on its own"""
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", kfile) if kfile else None
-        if tpath and os.path.exists(tpath):
-            with open(tpath) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+    # per-kernel counters of the profiled step (rocprofv3; the file names its commands)
+    counters = {}
+    cpath = os.path.join(ROOT, "profiles", "step_3d128_fp32_counters.json")
+    if os.path.exists(cpath):
+        with open(cpath) as f:
+            counters = json.load(f).get("kernels", {})
+
+    def traffic_of(*stages):
+        vals = [counters[s_]["hbm_bytes"] for s_ in stages if s_ in counters and "hbm_bytes" in counters[s_]]
+        return sum(vals) if len(vals) == len(stages) else None
+
+    def roofline(kname, bpc, kms, traffic=None, alone_ms=None):
+        """achieved = algorithmic bytes (bpc per cell) / the stage's duration inside the
+        step (kms, HIP events on the engine stream); traffic: the HBM bytes rocprofv3
+        counted for it (FETCH_SIZE + WRITE_SIZE, calibrated)"""
         achieved = bpc * ncell / (kms * 1e-3) / 1e9
         r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname, "bytes_per_cell": bpc,
@@ -333,29 +397,62 @@ def main():
             r["frac_alone"] = bpc * ncell / (alone_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
         return r
 
-    roof_other = None
+    passes = None
     if split:
         kp_in = in_step["k1_potentials"] if in_step else kp_ms
         kf_in = in_step["k1_flux"] if in_step else kf_ms
-        passes = [roofline("k_potentials<float> (interpolation from the LDS node box + kappa lap + psi -> mu, Mob)",
-                           "k_potentials_3d128_fp32_traffic.json", 56, kp_in, kp_ms),
-                  roofline("k_fluxmarch<float> (face fluxes + divergence from X, mu, Mob; dt controller in its "
-                           "last block)" if march else "k_fluxdiv<float> (face fluxes + divergence)",
-                           "k_fluxmarch_3d128_fp32_traffic.json" if march else None, 68, kf_in, kf_ms)]
-        passes.sort(key=lambda r: -r["avg_ms"])
-        roof, roof_other = passes
+        roof = roofline("K1 stencil stage: k_potentials (interpolation from the LDS node box + kappa lap + psi "
+                        "-> mu, Mob) + k_fluxmarch (face fluxes + divergence + block max; dt controller in its "
+                        "last block)", 36, kp_in + kf_in, traffic_of("potentials", "flux"),
+                        alone_ms=(kp_ms + kf_ms) if kp_ms and kf_ms else None)
+        passes = [roofline("k_potentials (its bytes include the mu, Mob it writes)", 56, kp_in,
+                           traffic_of("potentials"), kp_ms),
+                  roofline("k_fluxmarch (its bytes include the mu, Mob it reads)", 68, kf_in, traffic_of("flux"),
+                           kf_ms)]
     else:
-        roof = roofline("k_stencil<float> (fused interpolation + kappa lap + mu + face flux + divergence)",
-                        "k_stencil_3d128_fp32_traffic.json", 36, k1_ms)
+        roof = roofline("K1 stencil stage: k_stencil (fused interpolation + kappa lap + mu + face flux + "
+                        "divergence)", 36, k1_ms)
+    step_ms = in_step["step"] if in_step else 1e3 * el32 / args.steps
+    roof_step = roofline("whole step (FFT pair, K1, dt, update)", 96, 1e3 * el32 / args.steps,
+                         traffic_of("fft_fwd", "fft_x", "fft_inv", "potentials", "flux", "update"))
+    roof_step["in_step_events_ms"] = step_ms
+    fft_in = in_step["fft"] if in_step else fft_ms
+    roof_fft = roofline("FFT pair (forward plane, x with Bhat, inverse plane passes)", 8, fft_in,
+                        traffic_of("fft_fwd", "fft_x", "fft_inv"))
+    # VALU issue fraction: instructions per launch (counters) over the in-step duration
+    peak_inst = 256 * 2.4e9
+    live = {"potentials": in_step["k1_potentials"] if in_step else kp_ms,
+            "flux": in_step["k1_flux"] if in_step else kf_ms,
+            "update": in_step["k2"] if in_step else k2_ms}
+    valu = {}
+    for stage_, c in counters.items():
+        if "SQ_INSTS_VALU" not in c:
+            continue
+        ms = live.get(stage_) or c.get("avg_us", 0) * 1e-3
+        valu[stage_] = {"kernel": c.get("kernel"), "insts": c["SQ_INSTS_VALU"], "avg_ms": ms,
+                        "valu_frac": c["SQ_INSTS_VALU"] / (ms * 1e-3 * peak_inst) if ms else None,
+                        "duration_from": "in-step HIP events" if stage_ in live else "profiled trace"}
+    roc = None
+    if not args.fp32_only:
+        roc = {"fp32_ms": rocfft_ms(pb, "fp32"), "fp64_ms": rocfft_ms(pb, "fp64"), "hand_fp32_in_step_ms": fft_in,
+               "hand_fp32_alone_ms": fft_ms, "hand_fp64_alone_ms": fft_64}
+        log("rocFFT pair", roc)
     out = {
         "metric": METRIC, "value": v32, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el32 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "3D 128^3 periodic explicit step, FP32 fused device path (FP64 mass, dt, k=0 mode), "
-                               "elasticity on, synthetic regular-solution table",
+                               "elasticity on, synthetic regular-solution table; value is one simulation (a batch "
+                               "of one), fp32_batched_128 holds eight in one engine",
                    "cells": ncell, "shape": list(pb.shape), "precision": "fp32", "elastic": True},
         "roofline": roof,
-        "roofline_other": roof_other,
+        "roofline_step": roof_step,
+        "roofline_fft": roof_fft,
+        "roofline_passes": passes,
+        "valu": valu,
+        "cache_note": "one 128^3 FP32 simulation's per-step working set (about 200 MB) fits the 256 MB Infinity "
+                      "Cache, so part of the traffic these fractions count against 8 TB/s HBM is served there",
+        "rocfft": roc,
         "fp64_fused": {"value": v64, "ms_per_step": 1e3 * el64 / args.steps if el64 else None, "k1_ms": k1_64},
         "fp64_plain": {"value": vplain, "steps": args.plain_steps},
         "ratio_fp32_vs_plain_fp64": v32 / vplain if vplain else None,
@@ -385,6 +482,8 @@ def main():
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(pb, seconds=args.cpu_seconds)
         log("cpu", out["cpu_baseline"])
+        out["cpu_baseline_1thread"] = cpu_baseline(pb, seconds=min(args.cpu_seconds, 10.0), threads=1)
+        log("cpu 1 thread", out["cpu_baseline_1thread"])
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -93,7 +93,10 @@ class Server:
         return req
 
     def _setup_problem(self, req):
-        if self.fast_init:
+        # audited requests take the host draw (apply_fluctuations under the request's
+        # seed, the reference's seeded MT19937 sequence) so that the audit starts from
+        # the reference's initial field; the others draw theirs on the device
+        if self.fast_init and not req.audit:
             # the initial field is drawn on the device when the request is admitted
             pb = build_problem(req.cfg, table=req.table, X_init=np.empty((0,)))
             pb.X_init = None
@@ -207,7 +210,8 @@ def poisson_requests(rate, duration, seed=0, max_steps=500, audit_frac=0.05, t_e
     """Arrivals at Poisson ``rate`` for ``duration`` s.  Dimension 1/2/3 with probability
     0.3/0.5/0.2: 1D mw 20-200 x ppw 8-16; 2D sides 128-512; 3D sides 64-256 (sides drawn
     as multiples of 16); composition and kappa as in the batched mix; end time 50 CTLD,
-    capped at ``max_steps`` CFL-limited steps when given."""
+    capped at ``max_steps`` CFL-limited steps when given.  Every (1 / audit_frac)-th
+    request is audited."""
     from .workloads import _mix_config, draw_spinodal_composition, table_for
     rng = np.random.default_rng(seed)
     t, reqs, rid = 0.0, [], 0
@@ -227,7 +231,7 @@ def poisson_requests(rate, duration, seed=0, max_steps=500, audit_frac=0.05, t_e
         else:
             cfg = _mix_config(3, 16 * int(rng.integers(4, 17)), X0, kappa, T, rid)
         reqs.append(Request(rid, cfg, table_for(T), t, t_end_ctld=t_end_ctld, max_steps=max_steps,
-                            audit=bool(rng.uniform() < audit_frac)))
+                            audit=audit_frac > 0 and rid % max(1, int(round(1.0 / audit_frac))) == 0))
         rid += 1
     return reqs
 
