@@ -175,7 +175,7 @@ int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts, int nt);
  * is 128: each 128-point line transformed in the registers of 8 threads of one
  * wavefront.  mode is a mask of the passes that use them: 1 the forward plane pass,
  * 2 the x pass, 4 the inverse plane pass (the others run the Stockham plane or line
- * passes above).  Defaults: 1 in FP32, 7 in FP64.  Every mode gives the same bits. */
+ * passes above).  Defaults: 3 in FP32, 7 in FP64.  Every mode gives the same bits. */
 int pf5d_engine_set_fft_wave(pf5d_engine* e, int mode);
 /* enqueue nsteps (graph replay); does not block */
 int pf5d_engine_step(pf5d_engine* e, int nsteps);

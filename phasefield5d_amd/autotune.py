@@ -24,7 +24,7 @@ _lock = threading.Lock()
 
 def signature(engine):
     shapes = sorted(tuple(pb.shape) for pb, _ in engine.slots.values())
-    return f"v5:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
+    return f"v6:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
 
 
 def _load():

@@ -367,13 +367,15 @@ template <typename T>
 static WaveKernels<T> wave_kernels() {
   // FP32: forward planes in blocks of 8 waves, inverse planes of 9 (all 65
   // spectrum columns in one round), 2 parts each; FP64 (twice the LDS per line):
-  // 4 waves, 2 parts; x lines in blocks of 4 waves
+  // 4 waves, 2 parts.  x lines in blocks of one wave: at 128^3 the pass has about
+  // one wave per SIMD, and one-wave blocks spread it over every CU (FP32 x pass
+  // 14.5 us in four-wave blocks, 10.3 us in one-wave blocks; two-wave blocks 11.2)
   if constexpr (sizeof(T) == 4)
     return {&k_frp_fwd<T, 8, 2>, &k_frp_inv<T, 9, 2>, FrPlaneFwd<T, 8, 2>::BYTES, FrPlaneInv<T, 9, 2>::BYTES,
-            8 * 64, 9 * 64, 2, 2, &k_frx<T, 4>, FrX<T, 4>::BYTES, 4 * 64};
+            8 * 64, 9 * 64, 2, 2, &k_frx<T, 1>, FrX<T, 1>::BYTES, 1 * 64};
   else
     return {&k_frp_fwd<T, 4, 2>, &k_frp_inv<T, 4, 2>, FrPlaneFwd<T, 4, 2>::BYTES, FrPlaneInv<T, 4, 2>::BYTES,
-            4 * 64, 4 * 64, 2, 2, &k_frx<T, 4>, FrX<T, 4>::BYTES, 4 * 64};
+            4 * 64, 4 * 64, 2, 2, &k_frx<T, 1>, FrX<T, 1>::BYTES, 1 * 64};
 }
 
 // the five stages of psi = irfftn(Bhat rfftn(s)) and their work items
@@ -1038,7 +1040,7 @@ struct Engine : EngineBase {
     fft.nt = 256;
     fft.parts = sizeof(T) == 4 ? 2 : 4;
     fft.pnt = 1024;
-    fft.wave = sizeof(T) == 4 ? 1 : FFT_WAVE_ALL;
+    fft.wave = sizeof(T) == 4 ? 3 : FFT_WAVE_ALL;  // (FP32: the Stockham inverse plane pass is faster)
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 64>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 512>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);
     (void)hipFuncSetAttribute((const void*)&k_fft<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize, devinfo().lds);

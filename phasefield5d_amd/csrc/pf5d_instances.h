@@ -43,8 +43,8 @@ namespace pf5d {
   PF5D_FRP_INSTANCE(float, 8, 2)  \
   PF5D_FRP_INSTANCE(float, 9, 2)  \
   PF5D_FRP_INSTANCE(double, 4, 2) \
-  PF5D_FRX_INSTANCE(float, 4)     \
-  PF5D_FRX_INSTANCE(double, 4)
+  PF5D_FRX_INSTANCE(float, 1)     \
+  PF5D_FRX_INSTANCE(double, 1)
 #define PF5D_STENCIL_INSTANCE(T, BY, BZ, NT, MINW)                                                 \
   PF5D_INST __global__ void k_stencil<T, BY, BZ, NT, MINW>(const SimDev<T>* __restrict__,          \
                                                            SimState* __restrict__, T* __restrict__, \
