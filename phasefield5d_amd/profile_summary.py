@@ -71,10 +71,9 @@ def summarise(d, steps):
         for r in read_csv(path):
             k = short_name(r["Kernel_Name"])
             per[(k, r["Dispatch_Id"], int(r["Grid_Size"]))][r["Counter_Name"]] = float(r["Counter_Value"])
-            meta[k] = {"vgpr": int(r["VGPR_Count"]), "agpr": int(r["Accum_VGPR_Count"]),
-                       "sgpr": int(r["SGPR_Count"]), "scratch_bytes": int(r["Scratch_Size"]),
-                       "lds_bytes": int(r["LDS_Block_Size"]), "workgroup": int(r["Workgroup_Size"]),
-                       "grid": int(r["Grid_Size"])}
+            # (rocprofv3's own fields, in its units)
+            meta[k] = {c: int(r[c]) for c in ("VGPR_Count", "Accum_VGPR_Count", "SGPR_Count", "Scratch_Size",
+                                              "LDS_Block_Size", "Workgroup_Size", "Grid_Size")}
         for (k, _, grid), cv in per.items():
             for c, v in cv.items():
                 counters[k][c].append(v)
@@ -131,7 +130,7 @@ def markdown(s, title, commands):
             "(the factors that make the run's 256 MiB device-to-device copy read and write 256 MiB). "
             f"valu_frac = SQ_INSTS_VALU / (duration x {s['valu_peak']}).", ""]
     cols = ("avg_us", "hbm_bytes", "SQ_INSTS_VALU", "valu_frac", "SQ_BUSY_CYCLES", "SQ_WAIT_ANY",
-            "SQ_LDS_BANK_CONFLICT", "SQ_WAVES", "vgpr", "scratch_bytes", "lds_bytes")
+            "SQ_LDS_BANK_CONFLICT", "SQ_WAVES", "VGPR_Count", "Scratch_Size", "LDS_Block_Size")
     out.append("| stage | kernel | " + " | ".join(cols) + " |")
     out.append("|---|---|" + "---|" * len(cols))
 
