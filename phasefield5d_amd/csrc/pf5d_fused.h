@@ -410,39 +410,45 @@ PF5D_DEV void dt_control(int sim, const SimDev<T>* __restrict__ sims, SimState* 
   // (uniform over the block: every thread reads the same state)
   const int iv = sims[sim].oob_interval;
   const bool reduce = !advance || (iv > 0 && (st[sim].steps % iv) == 0);
-  const int q1 = reduce ? p1 : p0;
-  // Neumaier sums of the mass partials, min/max of the rest
-  double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
-  double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
-  for (int i = p0 + tid; i < q1; i += NTB) {
-    const double* q = part + (long)i * NSTAT;
+  double v[NSTAT];
+  if (reduce) {  // (uniform over the block: every thread read the same state)
+    // Neumaier sums of the mass partials, min/max of the rest
+    double sum[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
+    double nf = 0, smax = -1e300, smin = 1e300, cmin = 1e300, cmax = -1e300;
+    for (int i = p0 + tid; i < p1; i += NTB) {
+      const double* q = part + (long)i * NSTAT;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const double v = q[a], t = sum[a] + v;
-      cmp[a] += (fabs(sum[a]) >= fabs(v)) ? ((sum[a] - t) + v) : ((v - t) + sum[a]);
-      sum[a] = t;
+      for (int a = 0; a < 4; ++a) {
+        const double u = q[a], t = sum[a] + u;
+        cmp[a] += (fabs(sum[a]) >= fabs(u)) ? ((sum[a] - t) + u) : ((u - t) + sum[a]);
+        sum[a] = t;
+      }
+      nf += q[4];
+      smax = fmax(smax, q[5]);
+      smin = fmin(smin, q[6]);
+      cmin = fmin(cmin, q[7]);
+      cmax = fmax(cmax, q[8]);
     }
-    nf += q[4];
-    smax = fmax(smax, q[5]);
-    smin = fmin(smin, q[6]);
-    cmin = fmin(cmin, q[7]);
-    cmax = fmax(cmax, q[8]);
-  }
-  double v[NSTAT] = {sum[0] + cmp[0], sum[1] + cmp[1], sum[2] + cmp[2], sum[3] + cmp[3], nf, smax, smin, cmin, cmax};
+    const double v0[NSTAT] = {sum[0] + cmp[0], sum[1] + cmp[1], sum[2] + cmp[2], sum[3] + cmp[3], nf, smax, smin,
+                              cmin, cmax};
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+    for (int k = 0; k < NSTAT; ++k) v[k] = v0[k];
 #pragma unroll
-    for (int a = 0; a < 5; ++a) v[a] += __shfl_xor(v[a], off, 64);
-    v[5] = fmax(v[5], __shfl_xor(v[5], off, 64));
-    v[6] = fmin(v[6], __shfl_xor(v[6], off, 64));
-    v[7] = fmin(v[7], __shfl_xor(v[7], off, 64));
-    v[8] = fmax(v[8], __shfl_xor(v[8], off, 64));
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int a = 0; a < 5; ++a) v[a] += __shfl_xor(v[a], off, 64);
+      v[5] = fmax(v[5], __shfl_xor(v[5], off, 64));
+      v[6] = fmin(v[6], __shfl_xor(v[6], off, 64));
+      v[7] = fmin(v[7], __shfl_xor(v[7], off, 64));
+      v[8] = fmax(v[8], __shfl_xor(v[8], off, 64));
+    }
+    if ((tid & 63) == 0)
+      for (int k = 0; k < NSTAT; ++k) red[tid >> 6][k] = v[k];
   }
-  if ((tid & 63) == 0)
-    for (int k = 0; k < NSTAT; ++k) red[tid >> 6][k] = v[k];
+  // (and every thread has read the state before thread 0 changes it below)
   __syncthreads();
   if (tid == 0) {
-    for (int w = 1; w < NTB / 64; ++w) {
+    for (int w = 1; w < NTB / 64 && reduce; ++w) {
       for (int a = 0; a < 5; ++a) v[a] += red[w][a];
       v[5] = fmax(v[5], red[w][5]);
       v[6] = fmin(v[6], red[w][6]);
