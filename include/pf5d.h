@@ -179,6 +179,11 @@ int pf5d_engine_set_fft_planes(pf5d_engine* e, int parts, int nt);
 int pf5d_engine_set_fft_wave(pf5d_engine* e, int mode);
 /* enqueue nsteps (graph replay); does not block */
 int pf5d_engine_step(pf5d_engine* e, int nsteps);
+/* capture (without running them) the step graphs pf5d_engine_step(e, nsteps) replays, so
+ * that the first such call does not pay for graph capture and instantiation (a timed
+ * region, a latency-bound loop).  No counterpart in the reference (its loop launches
+ * operators one by one). */
+int pf5d_engine_prepare(pf5d_engine* e, int nsteps);
 int pf5d_engine_sync(pf5d_engine* e);
 /* state[16] = dt, t, last_xmax, mass[4], steps, status, smax, smin, cmin, cmax, nonfinite, mean0, mean1 */
 int pf5d_engine_get_state(pf5d_engine* e, int slot, double* state16);

@@ -59,6 +59,7 @@ _SIGS = {
     "pf5d_engine_set_fft_planes": (i32, [vp, i32, i32]),
     "pf5d_engine_set_fft_wave": (i32, [vp, i32]),
     "pf5d_engine_step": (i32, [vp, i32]),
+    "pf5d_engine_prepare": (i32, [vp, i32]),
     "pf5d_engine_sync": (i32, [vp]),
     "pf5d_engine_get_state": (i32, [vp, i32, P_f64]),
     "pf5d_engine_set_time": (i32, [vp, i32, f64, f64]),

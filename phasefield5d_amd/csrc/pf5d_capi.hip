@@ -837,6 +837,7 @@ struct EngineBase {
                              const pf5d_table* tab, int* slot) = 0;
   virtual void remove(int slot) = 0;
   virtual void step(int n) = 0;
+  virtual void prepare(int n) = 0;
   virtual void get_state(int slot, double* out) = 0;
   virtual int get_all_states(int maxn, double* out) = 0;
   virtual void set_time(int slot, double dt, double t) = 0;
@@ -1595,7 +1596,8 @@ struct Engine : EngineBase {
     if (dirty) rebuild();
     return std::max<int>(1, (int)groups.size());
   }
-  void step(int n) override {
+  // the step graphs step(n) replays, captured (not run) ahead of it
+  void prepare(int n) override {
     require(n >= 0, "nsteps must be >= 0");
     if (dirty) rebuild();
     if (nk1 == 0 || n == 0) return;
@@ -1605,6 +1607,15 @@ struct Engine : EngineBase {
         if (!g.g1) g.g1 = capture_with(1, [&] { enqueue_group_step(g); });
         if (n >= GNG && !g.gN) g.gN = capture_with(GNG, [&] { enqueue_group_step(g); });
       }
+      return;
+    }
+    if (!g1) g1 = capture(1);
+    if (n >= GN && !gN) gN = capture(GN);
+  }
+  void step(int n) override {
+    prepare(n);
+    if (nk1 == 0 || n == 0) return;
+    if (groups.size() > 1) {
       while (n >= GNG) {
         for (auto& g : groups) HIPCHK(hipGraphLaunch(g->gN, stream));
         n -= GNG;
@@ -1614,8 +1625,6 @@ struct Engine : EngineBase {
       stats_stale = true;
       return;
     }
-    if (!g1) g1 = capture(1);
-    if (n >= GN && !gN) gN = capture(GN);
     while (n >= GN) {
       HIPCHK(hipGraphLaunch(gN, stream));
       n -= GN;
@@ -2322,6 +2331,9 @@ int pf5d_engine_set_fft(pf5d_engine* e, int budget_kb, int nt) {
 }
 int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx) {
   return guarded([&] { e->impl->set_tiling(variant, cx); });
+}
+int pf5d_engine_prepare(pf5d_engine* e, int nsteps) {
+  return guarded([&] { e->impl->prepare(nsteps); });
 }
 int pf5d_engine_step(pf5d_engine* e, int nsteps) {
   return guarded([&] { e->impl->step(nsteps); });

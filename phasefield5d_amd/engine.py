@@ -91,6 +91,10 @@ class Engine:
     def step(self, n=1):
         _lib.check(_lib.lib().pf5d_engine_step(self.h, int(n)))
 
+    def prepare(self, n=1):
+        """Capture the step graphs ``step(n)`` replays, without running them."""
+        _lib.check(_lib.lib().pf5d_engine_prepare(self.h, int(n)))
+
     def sync(self):
         _lib.check(_lib.lib().pf5d_engine_sync(self.h))
 

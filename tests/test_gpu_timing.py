@@ -76,3 +76,22 @@ def test_kernel_timing_leaves_trajectory_unchanged(dev, precision):
     assert st["steps"] == n and st["dt"] == st_ref["dt"] and st["t"] == st_ref["t"]
     assert np.array_equal(st["mass"], st_ref["mass"])
     assert np.array_equal(X, X_ref)
+
+
+def test_prepare_captures_without_stepping(dev):
+    """prepare(n) captures the graphs step(n) replays and runs none of them: the state is
+    untouched, and the steps after it are those of an engine that never prepared."""
+    pb = problem(3, 24, seed=4)
+    n = 20
+    X_ref, st_ref, _ = _run(pb, "fp32", lambda e: e.step(n))
+
+    def prepared(e):
+        X0 = e.field(0, "X")
+        e.prepare(n)
+        e.sync()
+        st0 = e.state(0)
+        assert st0["steps"] == 0 and np.array_equal(e.field(0, "X"), X0)
+        e.step(n)
+
+    X, st, _ = _run(pb, "fp32", prepared)
+    assert st["steps"] == n and st["dt"] == st_ref["dt"] and np.array_equal(X, X_ref)
