@@ -371,6 +371,39 @@ PF5D_DEV void get_cells(const Q16<T> (*src)[64], int lane, CellV<T> (&c)[CPL]) {
   }
 }
 
+// Whole-wavefront rotations by one lane (DPP, a VALU move: no LDS round trip):
+// rot_next gives lane i the value of lane (i + 1) mod 64, rot_prev that of lane
+// (i - 1) mod 64.  Every lane must be active.
+template <int CTRL>
+PF5D_DEV float dpp_rot(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+PF5D_DEV double dpp_rot(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+constexpr int DPP_WAVE_ROL1 = 0x134;  // wave_rol:1
+constexpr int DPP_WAVE_ROR1 = 0x13C;  // wave_ror:1
+template <typename T>
+PF5D_DEV T rot_next(T v) { return dpp_rot<DPP_WAVE_ROL1>(v); }
+template <typename T>
+PF5D_DEV T rot_prev(T v) { return dpp_rot<DPP_WAVE_ROR1>(v); }
+template <typename T>
+PF5D_DEV CellV<T> rot_next_cell(const CellV<T>& v) {
+  CellV<T> r;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    r.X.v[a] = rot_next<T>(v.X.v[a]);
+    r.mu.v[a] = rot_next<T>(v.mu.v[a]);
+  }
+#pragma unroll
+  for (int m = 0; m < 5; ++m) r.M[m] = rot_next<T>(v.M[m]);
+  return r;
+}
+
 // X, mu, Mob of lane `src` of the wavefront
 template <typename T>
 PF5D_DEV CellV<T> shfl_cell_from(const CellV<T>& v, int src) {
@@ -563,7 +596,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
         // the cell after the lane's last one: the next lane's first, or, for the
         // row's last cell, the row's first (periodic tile) or the cell after the tile
         const bool ends = lane == lend;
-        CellV<T> nb = shfl_cell_from<T>(me[0], (zwrap && ends) ? 0 : ((lane + 1) & 63));
+        // a row of 64 * CPL cells fills the wavefront: the next lane is a rotation
+        // (uniform over the block: the tile's row length)
+        const bool full = tz == 64 * CPL;
+        CellV<T> nb = full ? rot_next_cell<T>(me[0]) : shfl_cell_from<T>(me[0], (zwrap && ends) ? 0 : ((lane + 1) & 63));
         if (!zwrap && ends) nb = load_cell<T>(sd, row_base(x, gy) + (item.z0 + tz < n2 ? item.z0 + tz : 0));
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
@@ -588,7 +624,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
           T v = fzu[CPL - 1][a];
 #pragma unroll
           for (int j = 0; j + 1 < CPL; ++j) v = (ends && jend == j) ? fzu[j][a] : v;
-          fzd[0][a] = __shfl(v, (zwrap && lane == 0) ? lend : ((lane + 63) & 63), 64);
+          fzd[0][a] = full ? rot_prev<T>(v) : __shfl(v, (zwrap && lane == 0) ? lend : ((lane + 63) & 63), 64);
         }
         if (lane == 0 && !zwrap) {
           const CellV<T> dn = load_cell<T>(sd, row_base(x, gy) + (item.z0 > 0 ? item.z0 - 1 : n2 - 1));
