@@ -971,12 +971,23 @@ struct Engine : EngineBase {
   int nk1 = 0, nkf = 0, nk2 = 0, cap = 0;
   FftBatch<T> fft;
   bool dirty = true;
-  hipGraphExec_t g1 = nullptr, gN = nullptr;
-  static constexpr int GN = 8;
-  static constexpr int GNG = 16;  // steps per graph of a group (its fields stay cached meanwhile)
+  // Step graphs of 1, 2, 4, ... steps (graph gs[k]: 2^k steps).  A graph launch
+  // leaves the GPU idle for about 8 us (128^3: 6 % of a step) while a graph runs
+  // its kernels back to back, so n steps replay as many 2^GLOG-step graphs as
+  // fit, then the binary digits of the rest (20 steps: 16 + 4).
+  static constexpr int GLOG = 6;
+  static constexpr int GLOGG = 4;  // groups: at most 16 steps per group graph (its fields stay cached meanwhile)
+  hipGraphExec_t gs[GLOG + 1] = {};
+  // the graph launches of n steps: gs[lmax] while 2^lmax steps are left, then the binary digits
+  template <typename F>
+  static void for_chunks(int n, int lmax, F f) {
+    for (; n >= (1 << lmax); n -= 1 << lmax) f(lmax);
+    for (int k = lmax - 1; k >= 0; --k)
+      if (n & (1 << k)) f(k);
+  }
   unsigned k1_grid = 0, kf_grid = 0;
   // Simulations whose per-step working sets together exceed what the 256 MB
-  // Infinity Cache holds are stepped in groups: each group runs GNG steps (its
+  // Infinity Cache holds are stepped in groups: each group runs up to 16 steps (its
   // own graph) while its fields stay in the cache, then the next group.  Every
   // simulation advances by the same steps; only the order of independent work
   // changes.  A group is a range of the work lists, its own FFT batch and the
@@ -988,14 +999,15 @@ struct Engine : EngineBase {
     unsigned k1_grid = 0, kf_grid = 0, k2_grid = 0;
     int* d_list = nullptr;
     FftBatch<T> fft;
-    hipGraphExec_t g1 = nullptr, gN = nullptr;
+    hipGraphExec_t gs[GLOGG + 1] = {};
     Group() = default;
     Group(const Group&) = delete;  // owns device memory and graphs
     Group& operator=(const Group&) = delete;
     void drop() {
-      if (g1) (void)hipGraphExecDestroy(g1);
-      if (gN) (void)hipGraphExecDestroy(gN);
-      g1 = gN = nullptr;
+      for (auto& g : gs) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
     }
     ~Group() {
       drop();
@@ -1089,9 +1101,10 @@ struct Engine : EngineBase {
     s = Sim();
   }
   void drop_graphs() {
-    if (g1) (void)hipGraphExecDestroy(g1);
-    if (gN) (void)hipGraphExecDestroy(gN);
-    g1 = gN = nullptr;
+    for (auto& g : gs) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
     for (auto& g : groups) g->drop();
   }
   void ensure_capacity(int n) {
@@ -1604,32 +1617,28 @@ struct Engine : EngineBase {
     if (groups.size() > 1) {
       for (auto& gp : groups) {
         Group& g = *gp;
-        if (!g.g1) g.g1 = capture_with(1, [&] { enqueue_group_step(g); });
-        if (n >= GNG && !g.gN) g.gN = capture_with(GNG, [&] { enqueue_group_step(g); });
+        for_chunks(n, GLOGG, [&](int k) {
+          if (!g.gs[k]) g.gs[k] = capture_with(1 << k, [&] { enqueue_group_step(g); });
+        });
       }
       return;
     }
-    if (!g1) g1 = capture(1);
-    if (n >= GN && !gN) gN = capture(GN);
+    for_chunks(n, GLOG, [&](int k) {
+      if (!gs[k]) gs[k] = capture(1 << k);
+    });
   }
   void step(int n) override {
     prepare(n);
     if (nk1 == 0 || n == 0) return;
     if (groups.size() > 1) {
-      while (n >= GNG) {
-        for (auto& g : groups) HIPCHK(hipGraphLaunch(g->gN, stream));
-        n -= GNG;
-      }
-      for (auto& g : groups)
-        for (int i = 0; i < n; ++i) HIPCHK(hipGraphLaunch(g->g1, stream));
+      // each chunk: every group in turn (a group's fields stay cached through its chunk)
+      for_chunks(n, GLOGG, [&](int k) {
+        for (auto& g : groups) HIPCHK(hipGraphLaunch(g->gs[k], stream));
+      });
       stats_stale = true;
       return;
     }
-    while (n >= GN) {
-      HIPCHK(hipGraphLaunch(gN, stream));
-      n -= GN;
-    }
-    while (n-- > 0) HIPCHK(hipGraphLaunch(g1, stream));
+    for_chunks(n, GLOG, [&](int k) { HIPCHK(hipGraphLaunch(gs[k], stream)); });
     stats_stale = true;
   }
   // steps compute the FP64 statistics only ahead of bounds checks: bring them
