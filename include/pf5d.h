@@ -156,9 +156,13 @@ int pf5d_engine_add_generated(pf5d_engine* e, const pf5d_sim_desc* d, unsigned l
                               const double* bhat_host, const pf5d_table* tab, int* slot);
 int pf5d_engine_remove(pf5d_engine* e, int slot);
 /* K1 tile: variant index (pf5d_engine_num_variants), x chunk (3D), per dimension.
- * info (5*n ints, may be NULL), per variant: by, bz, nt (by == 0: two-pass form),
- * then the flux pass of the two-pass form: cells per lane (0: per-plane tiles,
- * > 0: marching along axis 0 in x chunks) and its block size */
+ * info (5*n ints, may be NULL), per variant: by, bz, nt (by == 0: two-pass form;
+ * by == -1: fused marching form, whose potentials pass covers only the marching
+ * tiles' boundary rows and planes), then the flux pass of the two-pass form:
+ * cells per lane (0: per-plane tiles, > 0: marching along axis 0 in x chunks)
+ * and its block size.  A variant that does not fit the simulations held at the
+ * next step (the fused form: 3D, rows of 128 cells, axis 1 a multiple of 8,
+ * axis 0 a multiple of the x chunk >= 3) runs the default tiling instead. */
 int pf5d_engine_num_variants(pf5d_engine* e, int* n, int* info);
 int pf5d_engine_set_tiling(pf5d_engine* e, int variant, int cx);
 /* FFT passes: LDS bytes per work item in KiB (sets how many lines one block

@@ -878,6 +878,10 @@ struct K1Variant {
   void (*fmarch)(const SimDev<T>*, SimState*, T*, const K1Item*, int, const double*, const int*, int, const int*,
                  unsigned*) = nullptr;
   int fminw = 0;  // marching pass: min waves per SIMD it was compiled for
+  // fused marching form (fmarch = k_stencilmarch): the potentials pass runs on
+  // the boundary rows and planes of the marching tiles only, the marching pass
+  // computes the other cells' mu and Mob itself
+  bool fused = false;
   // potentials pass with the nearest-row search for tables with missing rows
   void (*fpot_wide)(const SimDev<T>*, const SimState*, const K1Item*, int) = nullptr;
   bool split() const { return by == 0; }
@@ -908,6 +912,18 @@ static K1Variant<T> make_march_variant() {
   return v;
 }
 
+// fused marching form: k_stencilmarch (FNT threads: FNT/64 rows per tile) with
+// the potentials pass (NT threads, MINW waves per SIMD) on the tiles' boundary
+// rows and planes
+template <typename T, int NT, int MINW, int FNT, int FMINW>
+static K1Variant<T> make_fused_variant() {
+  K1Variant<T> v{0,   16,  NT, 0, nullptr, &k_potentials<T, NT, MINW, false>, nullptr, 2, FNT,
+                 &k_stencilmarch<T, FNT, FMINW>, FMINW};
+  v.fpot_wide = &k_potentials<T, NT, MINW, true>;
+  v.fused = true;
+  return v;
+}
+
 // stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
 template <typename T>
 static std::vector<K1Variant<T>> k1_variants();
@@ -928,7 +944,8 @@ std::vector<K1Variant<float>> k1_variants<float>() {
           make_split_variant<float, 256, 4, 1>(), make_split_variant<float, 256, 4, 1, 32>(),
           make_split_variant<float, 256, 4, 2>(), make_march_variant<float, 256, 4, 256, 1>(),
           make_march_variant<float, 256, 4, 256, 2>(), make_march_variant<float, 256, 4, 512, 2>(),
-          make_march_variant<float, 256, 4, 256, 2, 3>()};
+          make_march_variant<float, 256, 4, 256, 2, 3>(), make_fused_variant<float, 256, 4, 512, 2>(),
+          make_fused_variant<float, 256, 3, 512, 2>()};
 }
 
 constexpr int K2_NT = 256;
@@ -1308,6 +1325,69 @@ struct Engine : EngineBase {
   void (*pot(const K1Variant<T>& V) const)(const SimDev<T>*, const SimState*, const K1Item*, int) {
     return wide ? V.fpot_wide : V.fpot;
   }
+  // The fused marching form (k_stencilmarch) needs every simulation 3D with
+  // rows of 64 * fcpl cells, the other axes in whole tiles (fnt / 64 rows) and
+  // runs (xchunk >= 3 planes), and complete tables (the wide nearest-row search
+  // is in the potentials pass only).
+  bool fused_fits(const K1Variant<T>& V) const {
+    if (!V.fused || wide || xchunk < 3) return false;
+    const int rows = V.fnt / 64;
+    int n = 0;
+    for (const auto& s : sims) {
+      if (!s.used) continue;
+      ++n;
+      const pf5d_sim_desc& d = s.d;
+      if (d.dim != 3 || d.shape[2] != 64 * V.fcpl || d.shape[1] % rows != 0 || d.shape[0] % xchunk != 0)
+        return false;
+    }
+    return n > 0;
+  }
+  // Work lists of the fused marching form for simulation i: marching tiles of
+  // `rows` rows over runs of R planes (kf), and potentials items (k1) on the
+  // cells whose mu and Mob the marching pass reads but does not compute -- the
+  // runs' first and last planes (every row), and on the other planes the tiles'
+  // first and last rows.  Both lists in slabs of whole runs, one per XCD, in the
+  // order the other kernels' lists take them.
+  void fused_lists(int i, const SimDev<T>& v, const K1Variant<T>& V, std::vector<K1Item>& k1,
+                   std::vector<K1Item>& kf) const {
+    const int R = xchunk, rows = V.fnt / 64, n0 = v.n0, n1 = v.n1, n2 = v.n2;
+    const int nruns = n0 / R, nsl = std::min(NXCD, nruns);
+    const int tz = std::min(V.bz, n2), ry = V.nt / tz;  // potentials tiles of the boundary planes
+    for (int sl = 0; sl < nsl; ++sl) {
+      const int ka = sl * nruns / nsl, kb = (sl + 1) * nruns / nsl;
+      for (int k = ka; k < kb; ++k) {
+        const int xa = k * R;
+        // the plane below the run and its first plane (wrapping at x = 0)
+        auto planes = [&](int pa, int pb) {
+          for (int z0 = 0; z0 < n2; z0 += tz)
+            for (int y0 = 0; y0 < n1; y0 += ry) k1.push_back({i, pa, pb, y0, z0, std::min(ry, n1 - y0), tz});
+        };
+        if (xa > 0) {
+          planes(xa - 1, xa + 1);
+        } else {
+          planes(n0 - 1, n0);
+          planes(0, 1);
+        }
+        // the tiles' last and first rows on the run's inner planes: row pairs
+        // (rows*j - 1, rows*j), rows of n2 cells, in items of about the cells
+        // of a plane item (the persistent grid's blocks then take similar work)
+        const int per = std::max(1, 2 * (tz * ry) / (2 * n2));  // planes per row item
+        for (int x0 = xa + 1; x0 < xa + R - 1; x0 += per) {
+          const int x1 = std::min(x0 + per, xa + R - 1);
+          for (int y = rows - 1; y < n1; y += rows) {
+            if (y + 1 < n1) {
+              k1.push_back({i, x0, x1, y, 0, 2, n2});
+            } else {
+              k1.push_back({i, x0, x1, y, 0, 1, n2});
+              k1.push_back({i, x0, x1, 0, 0, 1, n2});
+            }
+          }
+        }
+      }
+      for (int y0 = 0; y0 < n1; y0 += rows)
+        for (int k = ka; k < kb; ++k) kf.push_back({i, k * R, k * R + R, y0, 0, rows, n2});
+    }
+  }
   void rebuild() {
     HIPCHK(hipStreamSynchronize(stream));  // buffers below may still be in use
     drop_graphs();
@@ -1316,6 +1396,8 @@ struct Engine : EngineBase {
       if (s.used && s.tab && !s.tab->complete) wide = true;
     // the single-pass tilings search complete tables only: fall back to the default two-pass form
     if (wide && !variants[variant].split()) variant = default_variant;
+    // the fused marching form takes only the shapes it was built for
+    if (variants[variant].fused && !fused_fits(variants[variant])) variant = default_variant;
     const K1Variant<T>& V = variants[variant];
     std::vector<SimDev<T>> devs(cap);
     std::vector<K1Item> k1, kf;
@@ -1355,7 +1437,9 @@ struct Engine : EngineBase {
       devs[i] = make_dev(s);
       const SimDev<T>& v = devs[i];
       const bool hx = v.dim == 3;
-      if (V.split()) {
+      if (V.fused) {
+        fused_lists(i, v, V, k1, kf);
+      } else if (V.split()) {
         // tiles of at most nt cells: rows of up to bz cells (nt in 1D), as many rows as fit
         // slabs of planes (one per XCD, in the order the other kernels' lists
         // take them), then tile columns, x fastest: a block's contiguous range
@@ -1472,7 +1556,9 @@ struct Engine : EngineBase {
     if (V.split()) {
       auto grid = [&](int o) { return xcd_grid(std::min(nk1, devinfo().cus * std::max(1, o))); };
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)pot(V), V.nt, 0));
-      k1_grid = grid(occ);
+      // (the fused form's potentials items are small and alike: one block each,
+      // and the dispatcher keeps the CUs evenly loaded)
+      k1_grid = V.fused ? xcd_grid(nk1) : grid(occ);
       occ = 1;
       const void* ff = V.fcpl > 0 ? (const void*)V.fmarch : (const void*)V.fflux;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ff, V.fnt, 0));
@@ -1480,7 +1566,8 @@ struct Engine : EngineBase {
       int occ_pot = 1;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_pot, (const void*)pot(V), V.nt, 0));
       for (auto& g : groups) {
-        g->k1_grid = xcd_grid(std::max(1, std::min(g->nk1, devinfo().cus * std::max(1, occ_pot))));
+        g->k1_grid = V.fused ? xcd_grid(g->nk1)
+                             : xcd_grid(std::max(1, std::min(g->nk1, devinfo().cus * std::max(1, occ_pot))));
         g->kf_grid = xcd_grid(std::max(1, std::min(g->nkf, devinfo().cus * std::max(1, occ))));
         g->k2_grid = xcd_grid(std::max(1, std::min(g->nk2, devinfo().cus * 8)));
       }
@@ -1753,7 +1840,7 @@ struct Engine : EngineBase {
   int num_variants(int* info) override {
     if (info)
       for (size_t i = 0; i < variants.size(); ++i) {
-        info[5 * i] = variants[i].by;
+        info[5 * i] = variants[i].fused ? -1 : variants[i].by;
         info[5 * i + 1] = variants[i].bz;
         info[5 * i + 2] = variants[i].nt;
         info[5 * i + 3] = variants[i].fcpl;

@@ -61,6 +61,11 @@ namespace pf5d {
                                                     T* __restrict__, const K1Item* __restrict__, int,       \
                                                     const double* __restrict__, const int* __restrict__, int, \
                                                     const int* __restrict__, unsigned* __restrict__);
+#define PF5D_SMARCH_INSTANCE(T, NT, MINW)                                                                   \
+  PF5D_INST __global__ void k_stencilmarch<T, NT, MINW>(const SimDev<T>* __restrict__, SimState* __restrict__,  \
+                                                        T* __restrict__, const K1Item* __restrict__, int,       \
+                                                        const double* __restrict__, const int* __restrict__, int, \
+                                                        const int* __restrict__, unsigned* __restrict__);
 #define PF5D_FLUX_INSTANCE(T, NT, LOADS)                                                                       \
   PF5D_INST __global__ void k_fluxdiv<T, NT, LOADS>(const SimDev<T>* __restrict__, const SimState* __restrict__, \
                                                     T* __restrict__, const K1Item* __restrict__, int);
@@ -126,6 +131,7 @@ namespace pf5d {
   PF5D_FLUXM_INSTANCE(float, 256, 1, 1)           \
   PF5D_FLUXM_INSTANCE(float, 256, 2, 1)           \
   PF5D_FLUXM_INSTANCE(float, 256, 2, 3)           \
-  PF5D_FLUXM_INSTANCE(float, 512, 2, 1)
+  PF5D_FLUXM_INSTANCE(float, 512, 2, 1)           \
+  PF5D_SMARCH_INSTANCE(float, 512, 2)
 
 }  // namespace pf5d

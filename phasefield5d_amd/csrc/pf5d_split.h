@@ -31,6 +31,64 @@
 
 namespace pf5d {
 
+// mu of one cell from its interpolated properties P, the sum acc of its
+// neighbours' X (x, then y, then z: each up + down, as calculate_laplacian
+// adds them) and the elastic potential psi: (acc - 2 dim x) / h^2, then
+// P - L kappa (+ psi l) (simulate.py:253-255)
+template <typename T>
+PF5D_DEV V4<T> cell_mu(const SimDev<T>& sd, const V4<T>& xc, const T acc[4], const T P[9], T psi) {
+  const T cdim = T(2 * sd.dim);
+  V4<T> mu;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const T L = mad<T>(-cdim, xc.v[a], acc[a]) * sd.inv_dx2;
+    T m = mad<T>(-L, sd.kappa[a], P[a]);
+    if (sd.elastic) m = mad<T>(psi, sd.l[a], m);
+    mu.v[a] = m;
+  }
+  return mu;
+}
+
+// Stages the 4^4 node box of an item in LDS (BOX, BoxLayout<T>::SLOTS
+// 16-byte slots): the box around the centre cell of the item's planes (the
+// anchor), lower nodes one below its lower cube corners, so corners within +-1
+// of the anchor's are covered; lo[] receives the box's lower node.  Every
+// thread of the block calls it; the caller synchronises before reading BOX.
+template <typename T, int NT>
+PF5D_DEV void stage_box4(const SimDev<T>& sd, const K1Item& item, T* BOX, int lo[4], int tid) {
+  constexpr int NP = TablePad<T>::value;
+  constexpr int NBOX = BOX4 * BOX4 * BOX4 * BOX4;
+  using VT = typename Vec16<T>::type;
+  using L = BoxLayout<T>;
+  constexpr int NCOL = L::NCOL;  // 16-byte columns per table row
+  const int num = sd.tab.num;
+  {
+    const long ca = ((long)(item.xa + (item.xb - 1 - item.xa) / 2) * sd.n1 + item.y0 + item.ty / 2) * sd.n2 +
+                    item.z0 + item.tz / 2;
+    const V4<T> xa = ld4g<T>(sd.X + 4 * ca);
+    // (the box position only decides which points read LDS: clamped in
+    // floating point first, so NaN lands on 0 and the conversion is in range)
+    const float top = (float)(num - BOX4);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) lo[a] = (int)fminf(fmaxf((float)(xa.v[a] * sd.tab.inv_step) - 1.0f, 0.0f), top);
+  }
+  // one node per thread, all of its columns: the box's first row (64-bit),
+  // then offsets within the box (below 4 num^3 NP, 32-bit)
+  const T* row0 = sd.tab.data + ((((long)lo[0] * num + lo[1]) * num + lo[2]) * num + lo[3]) * NP;
+  const int m2 = num * num, m3 = m2 * num;
+#pragma unroll 1
+  for (int node = tid; node < NBOX; node += NT) {
+    const int q0 = node >> 6, q1 = (node >> 4) & 3, q2 = (node >> 2) & 3, q3 = node & 3;
+    const T* src = row0 + (q0 * m3 + q1 * m2 + q2 * num + q3) * NP;
+    VT* dst = reinterpret_cast<VT*>(BOX) + (q0 * L::S0 + q1 * L::S1 + q2 * L::S2 + q3 * L::S3);
+    VT v[NCOL];
+#pragma unroll
+    for (int col = 0; col < NCOL; ++col) v[col] = gld<VT>(src + col * Vec16<T>::n);
+#pragma unroll
+    for (int col = 0; col < NCOL; ++col) dst[col] = v[col];
+  }
+}
+
 // cell index of the tile thread, and whether it has a cell
 template <typename T>
 PF5D_DEV bool tile_cell(const SimDev<T>& sd, const K1Item& item, int tid, float rtz, int& gx, int& gy, int& gz) {
@@ -48,11 +106,7 @@ template <typename T, int NT, int MINW, bool WIDE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st, const K1Item* __restrict__ items,
              int nitems) {
-  constexpr int NP = TablePad<T>::value;
-  constexpr int NBOX = BOX4 * BOX4 * BOX4 * BOX4;
-  using VT = typename Vec16<T>::type;
   using L = BoxLayout<T>;
-  constexpr int NCOL = L::NCOL;  // 16-byte columns per table row
   __shared__ __align__(16) T BOX[L::SLOTS * Vec16<T>::n];
   int ib, ie;
   block_range(nitems, ib, ie);
@@ -71,39 +125,8 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
     long c = ((long)gx * n1 + gy) * n2 + gz;
     V4<T> xc{};
     if (valid) xc = ld4g<T>(sd.X + 4 * c);
-    // node box around the centre cell of the item's planes (the anchor): lower
-    // nodes one below its lower cube corners, so corners within +-1 of the
-    // anchor's are covered
-    const int num = sd.tab.num;
     int lo[4];
-    {
-      const long ca = ((long)(item.xa + (item.xb - 1 - item.xa) / 2) * n1 + item.y0 + item.ty / 2) * n2 + item.z0 +
-                      item.tz / 2;
-      const V4<T> xa = ld4g<T>(sd.X + 4 * ca);
-      // (the box position only decides which points read LDS: clamped in
-      // floating point first, so NaN lands on 0 and the conversion is in range)
-      const float top = (float)(num - BOX4);
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-        lo[a] = (int)fminf(fmaxf((float)(xa.v[a] * sd.tab.inv_step) - 1.0f, 0.0f), top);
-    }
-    {
-      // one node per thread, all of its columns: the box's first row (64-bit),
-      // then offsets within the box (below 4 num^3 NP, 32-bit)
-      const T* row0 = sd.tab.data + ((((long)lo[0] * num + lo[1]) * num + lo[2]) * num + lo[3]) * NP;
-      const int n2 = num * num, n3 = n2 * num;
-#pragma unroll 1
-      for (int node = tid; node < NBOX; node += NT) {
-        const int q0 = node >> 6, q1 = (node >> 4) & 3, q2 = (node >> 2) & 3, q3 = node & 3;
-        const T* src = row0 + (q0 * n3 + q1 * n2 + q2 * num + q3) * NP;
-        VT* dst = reinterpret_cast<VT*>(BOX) + (q0 * L::S0 + q1 * L::S1 + q2 * L::S2 + q3 * L::S3);
-        VT v[NCOL];
-#pragma unroll
-        for (int col = 0; col < NCOL; ++col) v[col] = gld<VT>(src + col * Vec16<T>::n);
-#pragma unroll
-        for (int col = 0; col < NCOL; ++col) dst[col] = v[col];
-      }
-    }
+    stage_box4<T, NT>(sd, item, BOX, lo, tid);
     __syncthreads();
     // the item's planes xa .. xb-1 share the staged box: one cell per plane.
     // FP32 walks up in x with the cell below (xm) and the cell itself carried
@@ -149,17 +172,10 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
         for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
       }
       const T psi = sd.elastic ? gld<T>(sd.psi + c) : T(0);
-      const T cdim = T(2 * sd.dim);
-      V4<T> mu, mb;
+      const V4<T> mu = cell_mu<T>(sd, xc, acc, P, psi);
+      V4<T> mb;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        // (acc - 2 dim x) / h^2, then P - L kappa (+ psi l): simulate.py:253-255
-        const T L = mad<T>(-cdim, xc.v[a], acc[a]) * sd.inv_dx2;
-        T m = mad<T>(-L, sd.kappa[a], P[a]);
-        if (sd.elastic) m = mad<T>(psi, sd.l[a], m);
-        mu.v[a] = m;
-        mb.v[a] = P[4 + a];
-      }
+      for (int a = 0; a < 4; ++a) mb.v[a] = P[4 + a];
       gst_field<T, V4<T>>(sd.mu + 4 * c, mu);
       gst_field<T, V4<T>>(sd.mb + 4 * c, mb);
       gst_field<T, T>(sd.mb1 + c, P[8]);
@@ -685,6 +701,327 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
       // (never true, since a stored maximum is a non-negative value whose bits
       // are not all ones) could be folded away, the maxima's atomics would lose
       // their returns and nothing would order them before the count.
+      asm volatile("" : "+v"(folded));
+      const unsigned one = 1u + (folded == ~0ull ? 1u : 0u);
+      last = atomicAdd(done, one) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last) {
+      for (int s = 0; s < ndt; ++s)
+        dt_control<T, NT>(dt_list ? dt_list[s] : s, sims, st, xmax_acc, part, part_range, 1, DRED, tid);
+      if (tid == 0) atomicExch(done, 0u);
+    }
+  }
+}
+
+
+// X of a lane's cells, packed for the LDS exchange of one plane's X values
+template <typename T, int CPL, int QX>
+PF5D_DEV void put_x(Q16<T> (*dst)[64], int lane, const V4<T> (&x)[CPL]) {
+  constexpr int QN = 16 / sizeof(T);
+  T p[QX * QN];
+#pragma unroll
+  for (int j = 0; j < CPL; ++j)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) p[4 * j + a] = x[j].v[a];
+#pragma unroll
+  for (int i = 4 * CPL; i < QX * QN; ++i) p[i] = T(0);
+#pragma unroll
+  for (int q = 0; q < QX; ++q) {
+    Q16<T> v;
+#pragma unroll
+    for (int e = 0; e < QN; ++e) v.v[e] = p[q * QN + e];
+    sst<Q16<T>>(&dst[q][lane], v);
+  }
+}
+template <typename T, int CPL, int QX>
+PF5D_DEV void get_x(const Q16<T> (*src)[64], int lane, V4<T> (&x)[CPL]) {
+  constexpr int QN = 16 / sizeof(T);
+  T p[QX * QN];
+#pragma unroll
+  for (int q = 0; q < QX; ++q) {
+    const Q16<T> v = sld<Q16<T>>(&src[q][lane]);
+#pragma unroll
+    for (int e = 0; e < QN; ++e) p[q * QN + e] = v.v[e];
+  }
+#pragma unroll
+  for (int j = 0; j < CPL; ++j)
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[j].v[a] = p[4 * j + a];
+}
+
+// Fused marching stencil stage: interpolation, kappa lap(X), mu, face fluxes
+// and divergence in one pass, for 3D fields whose rows (axis 2) hold 128
+// cells (two per lane of a wavefront).
+//
+// An item is a tile of NW rows (one per wavefront) over a run of planes
+// [xa, xb) on axis 0, stepped one plane at a time as in k_fluxmarch: the x
+// face below a plane is carried from the previous plane, the y faces are
+// exchanged between the wavefronts in LDS, the z faces between lanes.  The
+// values a plane's faces need -- X, mu and Mob of its cells and of the next
+// plane's -- come from two places:
+// - the tile's inner rows (1 .. NW-2) on the run's inner planes (xa+1 ..
+//   xb-2) are computed here, one plane ahead: interpolation from the item's
+//   staged node box, the Laplacian of X with its y neighbours from the other
+//   wavefronts (LDS, one plane ahead) and its z neighbours from the adjacent
+//   lanes, and psi;
+// - every other cell the tile reads -- its first and last rows, the run's first
+//   and last planes, the plane below the run and above it, the rows beside the
+//   tile -- lies on a boundary row (index 0 or NW-1 modulo NW) or a boundary
+//   plane (xa - 1 or xa modulo the run length) of the simulation, whose mu and
+//   Mob the potentials pass wrote (its work list holds only those cells).
+// Each cell's mu and Mob are computed once, by one of the two kernels, with
+// the same arithmetic (cell_mu, interp9_box4), so the result has the bits of
+// the two-pass form; only the boundary cells' mu and Mob pass through memory.
+// Items must hold NW rows of 128 cells and at least three planes; the host
+// checks the shape.  The dt controller runs in the last block as in
+// k_fluxmarch.
+template <typename T, int NT, int MINW>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
+k_stencilmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __restrict__ xmax_acc,
+               const K1Item* __restrict__ items, int nitems, const double* __restrict__ part,
+               const int* __restrict__ part_range, int ndt, const int* __restrict__ dt_list,
+               unsigned* __restrict__ done) {
+  constexpr int CPL = 2;
+  constexpr int NW = NT / 64;
+  constexpr int QN = 16 / sizeof(T);
+  constexpr int QC = (13 * CPL + QN - 1) / QN;  // 16-byte words per lane: cells
+  constexpr int QF = (4 * CPL + QN - 1) / QN;   // 16-byte words per lane: one face, or X, per cell
+  using L = BoxLayout<T>;
+  static_assert(NW >= 3, "a tile needs inner rows");
+  __shared__ __align__(16) T BOX[L::SLOTS * Vec16<T>::n];
+  __shared__ Q16<T> SC[NW][QC][64];
+  __shared__ Q16<T> SF[NW][QF][64];
+  __shared__ Q16<T> SX[2][NW][QF][64];  // X of one plane, every row (by plane parity)
+  __shared__ T RED[NW];
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int w = tid >> 6, lane = tid & 63;
+  int ib, ie;
+  block_range(nitems, ib, ie);
+  T tmax = T(0);
+  int cur = -1;
+  unsigned long long folded = 0;  // previous bits returned by the maxima's atomics
+  auto flush = [&](int sim) {
+    const T m = wave_max<T>(tmax);
+    if (lane == 0) RED[w] = m;
+    __syncthreads();
+    if (tid == 0) {
+      T r = RED[0];
+      for (int v = 1; v < NW; ++v) r = (RED[v] > r) ? RED[v] : r;
+      folded |= atomic_max_nonneg(&xmax_acc[sim], r);
+    }
+    __syncthreads();
+    tmax = T(0);
+  };
+  for (int it = ib; it < ie; ++it) {
+    const K1Item item = items[it];
+    if (item.sim != cur) {
+      if (cur >= 0) flush(cur);
+      cur = item.sim;
+    }
+    if (!(st[item.sim].status & ST_ACTIVE)) continue;  // uniform per block
+    const SimDev<T>& sd = sims[item.sim];
+    const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
+    const T inv_dx = sd.inv_dx, dscale = step_div_scale<T>(inv_dx);
+    const int xa = item.xa, xb = item.xb, run = xb - xa;
+    const int gy = item.y0 + w;
+    const bool inner = w > 0 && w < NW - 1;  // rows whose potentials are computed here
+    auto xw = [&](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
+    auto row_base = [&](int x, int y) { return ((long)x * n1 + y) * n2 + CPL * lane; };
+    auto load_full = [&](int x, int y, CellV<T>(&c)[CPL]) {
+      const long r = row_base(x, y);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) c[j] = load_cell<T>(sd, r + j);
+    };
+    auto load_x = [&](int x, V4<T>(&v)[CPL]) {
+      const long r = row_base(x, gy);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) v[j] = ld4g<T>(sd.X + 4 * (r + j));
+    };
+    // the row's psi at plane x, for the inner rows' potentials
+    auto load_psi = [&](int x, T(&p)[CPL]) {
+      const long r = row_base(x, gy);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) p[j] = gld<T>(sd.psi + r + j);
+    };
+    int lo[4];
+    stage_box4<T, NT>(sd, item, BOX, lo, tid);
+    CellV<T> me[CPL];
+    T fxd[CPL][4];
+    load_full(xa, gy, me);
+    {
+      CellV<T> pv[CPL];
+      load_full(xw(xa - 1), gy, pv);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) face_ab<T>(pv[j], me[j], inv_dx, fxd[j]);
+    }
+    // X of the next two planes (the next plane's cells and its x neighbour
+    // above), and psi of the next plane
+    V4<T> xn[CPL], xnn[CPL];
+    T ps[CPL] = {};
+    load_x(xw(xa + 1), xn);
+    load_x(xw(xa + 2), xnn);
+    if (inner && sd.elastic && run > 2) load_psi(xw(xa + 1), ps);
+    put_x<T, CPL, QF>(SX[(xa + 1) & 1][w], lane, xn);
+    __syncthreads();  // the node box and the next plane's X are in LDS
+#pragma unroll 1
+    for (int x = xa; x < xb; ++x) {
+      const int r1 = x + 1 - xa;  // the next plane's place in the run
+      const bool comp = inner && r1 <= run - 2;
+      // loads for the planes ahead: X of plane x + 3, psi of plane x + 2
+      V4<T> x3[CPL];
+      T ps2[CPL] = {};
+      if (x + 3 <= xb) load_x(xw(x + 3), x3);
+      if (inner && sd.elastic && r1 + 1 <= run - 2) load_psi(xw(x + 2), ps2);
+      CellV<T> nx[CPL];
+      CellV<T> hal[CPL];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) nx[j].X = xn[j];
+      if (comp) {
+        // the next plane's mu and Mob: its X with the y neighbours from the
+        // other rows, the z neighbours from the adjacent lanes (a row fills the
+        // wavefront: periodic), the x neighbours from this plane and the one after
+        V4<T> yu[CPL], yd[CPL], zu[CPL], zd[CPL];
+        get_x<T, CPL, QF>(SX[(x + 1) & 1][w + 1], lane, yu);
+        get_x<T, CPL, QF>(SX[(x + 1) & 1][w - 1], lane, yd);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          zu[0].v[a] = xn[1].v[a];
+          zd[0].v[a] = rot_prev<T>(xn[1].v[a]);
+          zu[1].v[a] = rot_next<T>(xn[0].v[a]);
+          zd[1].v[a] = xn[0].v[a];
+        }
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          T P[9];
+          interp9_box4<T, false>(sd.tab, BOX, lo, xn[j].v, P);
+          T acc[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            acc[a] = xnn[j].v[a] + me[j].X.v[a];
+            acc[a] = (acc[a] + yu[j].v[a]) + yd[j].v[a];
+            acc[a] = (acc[a] + zu[j].v[a]) + zd[j].v[a];
+          }
+          nx[j].mu = cell_mu<T>(sd, xn[j], acc, P, ps[j]);
+#pragma unroll
+          for (int m = 0; m < 5; ++m) nx[j].M[m] = P[4 + m];
+        }
+      } else {
+        // boundary rows and planes: the potentials pass wrote them
+        const long r = row_base(xw(x + 1), gy);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          nx[j].mu = ld4g<T>(sd.mu + 4 * (r + j));
+          const V4<T> m = ld4g<T>(sd.mb + 4 * (r + j));
+#pragma unroll
+          for (int a = 0; a < 4; ++a) nx[j].M[a] = m.v[a];
+          nx[j].M[4] = gld<T>(sd.mb1 + r + j);
+        }
+        // the rows beside the tile, for the edge rows' y faces
+        if (w == NW - 1)
+          load_full(x, gy + 1 < n1 ? gy + 1 : 0, hal);
+        else if (w == 0)
+          load_full(x, gy > 0 ? gy - 1 : n1 - 1, hal);
+      }
+      // the X of plane x + 2 for the next plane's inner rows, and this plane's
+      // cells for the y faces
+      if (r1 + 1 <= run - 2) put_x<T, CPL, QF>(SX[(x + 2) & 1][w], lane, xnn);
+      put_cells<T, CPL, QC>(SC[w], lane, me);
+      __syncthreads();
+      T fyu[CPL][4], fyd[CPL][4];
+      {
+        CellV<T> up[CPL];
+        if (w + 1 < NW)
+          get_cells<T, CPL, QC>(SC[w + 1], lane, up);
+        else {
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) up[j] = hal[j];
+        }
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], up[j], inv_dx, fyu[j]);
+      }
+      {
+        T p[QF * QN];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+#pragma unroll
+          for (int a = 0; a < 4; ++a) p[4 * j + a] = fyu[j][a];
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+          Q16<T> v;
+#pragma unroll
+          for (int e = 0; e < QN; ++e) v.v[e] = p[q * QN + e];
+          sst<Q16<T>>(&SF[w][q][lane], v);
+        }
+      }
+      __syncthreads();
+      if (w > 0) {
+        T p[QF * QN];
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+          const Q16<T> v = sld<Q16<T>>(&SF[w - 1][q][lane]);
+#pragma unroll
+          for (int e = 0; e < QN; ++e) p[q * QN + e] = v.v[e];
+        }
+#pragma unroll
+        for (int j = 0; j < CPL; ++j)
+#pragma unroll
+          for (int a = 0; a < 4; ++a) fyd[j][a] = p[4 * j + a];
+      } else {
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) face_ab<T>(hal[j], me[j], inv_dx, fyd[j]);
+      }
+      // z: the face above each cell (the row's last cell: its first, periodic),
+      // then the face below from the cell before
+      T fzu[CPL][4], fzd[CPL][4];
+      {
+        const CellV<T> nb = rot_next_cell<T>(me[0]);
+        face_ab<T>(me[0], me[1], inv_dx, fzu[0]);
+        face_ab<T>(me[1], nb, inv_dx, fzu[1]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          fzd[1][a] = fzu[0][a];
+          fzd[0][a] = rot_prev<T>(fzu[1][a]);
+        }
+      }
+      T fxu[CPL][4];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], nx[j], inv_dx, fxu[j]);
+      const long r = row_base(x, gy);
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        // divergence in the reference order: x, y, z, each (F+ - F-) added in turn
+        V4<T> o;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          T acc = fxu[j][a] - fxd[j][a];
+          acc = (acc + fyu[j][a]) - fyd[j][a];
+          acc = (acc + fzu[j][a]) - fzd[j][a];
+          o.v[a] = acc * dscale;
+          tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
+        }
+        gst_field<T, V4<T>>(sd.dX + 4 * (r + j), o);
+      }
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) fxd[j][a] = fxu[j][a];
+        me[j] = nx[j];
+        xn[j] = xnn[j];
+        xnn[j] = x3[j];
+        ps[j] = ps2[j];
+      }
+    }
+    __syncthreads();  // BOX, SX are reused by the next item
+  }
+  if (cur >= 0) flush(cur);
+  if (ndt > 0) {
+    __shared__ int last;
+    __shared__ double DRED[NW][NSTAT];
+    if (tid == 0) {
+      // as in k_fluxmarch: the count is made after the maxima's atomics, whose
+      // returns it depends on (hidden from the compiler by the empty asm)
       asm volatile("" : "+v"(folded));
       const unsigned one = 1u + (folded == ~0ull ? 1u : 0u);
       last = atomicAdd(done, one) == gridDim.x - 1;
