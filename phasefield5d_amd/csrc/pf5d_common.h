@@ -98,6 +98,24 @@ PF5D_DEV void sst(void* p, const V& v) {
   __builtin_memcpy(&w, &v, sizeof(V));
   *(PF5D_LDS W*)p = w;
 }
+// Accesses at a 32-bit byte offset from a base the whole wavefront shares
+// (a simulation's field, in scalar registers): the compiler then issues
+// global_load/store with the base as saddr and the offset in one VGPR, with no
+// 64-bit address arithmetic per access.  Offsets stay below 4 GiB (fields of
+// fewer than 2^27 FP64 cells).
+template <typename V>
+PF5D_DEV V gldo(const void* base, unsigned off) {
+  return gld<V>(static_cast<const char*>(base) + off);
+}
+template <typename V>
+PF5D_DEV void gsto(void* base, unsigned off, const V& v) {
+  gst<V>(static_cast<char*>(base) + off, v);
+}
+template <typename T, typename V>
+PF5D_DEV void gst_field_o(void* base, unsigned off, const V& v) {
+  gst_field<T, V>(static_cast<char*>(base) + off, v);
+}
+
 template <typename T>
 PF5D_DEV V4<T> ld4g(const T* p) { return gld<V4<T>>(p); }
 template <typename T>

@@ -67,10 +67,13 @@ PF5D_DEV void stage_box4(const SimDev<T>& sd, const K1Item& item, T* BOX, int lo
                     item.z0 + item.tz / 2;
     const V4<T> xa = ld4g<T>(sd.X + 4 * ca);
     // (the box position only decides which points read LDS: clamped in
-    // floating point first, so NaN lands on 0 and the conversion is in range)
+    // floating point first, so NaN lands on 0 and the conversion is in range;
+    // every thread computes the same box, and readfirstlane keeps it in scalar
+    // registers, where the per-point box tests read it)
     const float top = (float)(num - BOX4);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) lo[a] = (int)fminf(fmaxf((float)(xa.v[a] * sd.tab.inv_step) - 1.0f, 0.0f), top);
+    for (int a = 0; a < 4; ++a)
+      lo[a] = __builtin_amdgcn_readfirstlane((int)fminf(fmaxf((float)(xa.v[a] * sd.tab.inv_step) - 1.0f, 0.0f), top));
   }
   // one node per thread, all of its columns: the box's first row (64-bit),
   // then offsets within the box (below 4 num^3 NP, 32-bit)
@@ -204,6 +207,20 @@ PF5D_DEV CellV<T> load_cell(const SimDev<T>& sd, long c) {
 #pragma unroll
   for (int a = 0; a < 4; ++a) r.M[a] = m.v[a];
   r.M[4] = gld<T>(sd.mb1 + c);
+  return r;
+}
+// the same for a cell index below 2^32 / (4 sizeof(T)), as byte offsets from
+// the fields' bases (see gldo)
+template <typename T>
+PF5D_DEV CellV<T> load_cell32(const SimDev<T>& sd, unsigned c) {
+  CellV<T> r;
+  const unsigned o4 = c * (unsigned)sizeof(V4<T>), o1 = c * (unsigned)sizeof(T);
+  r.X = gldo<V4<T>>(sd.X, o4);
+  r.mu = gldo<V4<T>>(sd.mu, o4);
+  const V4<T> m = gldo<V4<T>>(sd.mb, o4);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) r.M[a] = m.v[a];
+  r.M[4] = gldo<T>(sd.mb1, o1);
   return r;
 }
 
@@ -508,7 +525,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
     const int tz = item.tz;
     const bool zwrap = tz == n2, ywrap = item.ty == n1;
     const int lend = (tz - 1) / CPL, jend = (tz - 1) - lend * CPL;  // lane and slot of the row's last cell
-    int kz[CPL];
+    unsigned kz[CPL];
     bool kon[CPL];
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
@@ -516,11 +533,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
       kon[j] = row_on && k < tz;
       kz[j] = item.z0 + min(k, tz - 1);
     }
-    auto row_base = [&](int x, int y) { return ((long)x * n1 + y) * n2; };
+    // cell indices in 32 bits (the host keeps fields below 2^27 cells)
+    auto row_base = [&](int x, int y) { return ((unsigned)x * n1 + y) * n2; };
     auto load_lane = [&](int x, int y, CellV<T>(&c)[CPL]) {
-      const long r = row_base(x, y);
+      const unsigned r = row_base(x, y);
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) c[j] = load_cell<T>(sd, r + kz[j]);
+      for (int j = 0; j < CPL; ++j) c[j] = load_cell32<T>(sd, r + kz[j]);
     };
     CellV<T> me[CPL];
     T fxd[CPL][4];
@@ -608,7 +626,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
       }
       // z: the face above each cell, then the face below from the cell before
       T fzu[CPL][4], fzd[CPL][4];
-      {
+      if (tz == 64 * CPL && zwrap) {
+        // a periodic row of 64 * CPL cells fills the wavefront (uniform over the
+        // block): the cell after the lane's last is the next lane's first, a
+        // rotation; the same faces as the general form below
+        const CellV<T> nb = rot_next_cell<T>(me[0]);
+#pragma unroll
+        for (int j = 0; j + 1 < CPL; ++j) face_ab<T>(me[j], me[j + 1], inv_dx, fzu[j]);
+        face_ab<T>(me[CPL - 1], nb, inv_dx, fzu[CPL - 1]);
+#pragma unroll
+        for (int j = 1; j < CPL; ++j)
+#pragma unroll
+          for (int a = 0; a < 4; ++a) fzd[j][a] = fzu[j - 1][a];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) fzd[0][a] = rot_prev<T>(fzu[CPL - 1][a]);
+      } else {
         // the cell after the lane's last one: the next lane's first, or, for the
         // row's last cell, the row's first (periodic tile) or the cell after the tile
         const bool ends = lane == lend;
@@ -616,7 +648,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
         // (uniform over the block: the tile's row length)
         const bool full = tz == 64 * CPL;
         CellV<T> nb = full ? rot_next_cell<T>(me[0]) : shfl_cell_from<T>(me[0], (zwrap && ends) ? 0 : ((lane + 1) & 63));
-        if (!zwrap && ends) nb = load_cell<T>(sd, row_base(x, gy) + (item.z0 + tz < n2 ? item.z0 + tz : 0));
+        if (!zwrap && ends) nb = load_cell32<T>(sd, row_base(x, gy) + (item.z0 + tz < n2 ? item.z0 + tz : 0));
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
           const bool last = ends && j == jend;
@@ -643,7 +675,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
           fzd[0][a] = full ? rot_prev<T>(v) : __shfl(v, (zwrap && lane == 0) ? lend : ((lane + 63) & 63), 64);
         }
         if (lane == 0 && !zwrap) {
-          const CellV<T> dn = load_cell<T>(sd, row_base(x, gy) + (item.z0 > 0 ? item.z0 - 1 : n2 - 1));
+          const CellV<T> dn = load_cell32<T>(sd, row_base(x, gy) + (item.z0 > 0 ? item.z0 - 1 : n2 - 1));
           face_ab<T>(dn, me[0], inv_dx, fzd[0]);
         }
       }
@@ -652,7 +684,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
 #pragma unroll
         for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], nx[j], inv_dx, fxu[j]);
       }
-      const long r = row_base(x, gy);
+      const unsigned r = row_base(x, gy);
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         // divergence in the reference order: x, y, z, each (F+ - F-) added in turn
@@ -676,7 +708,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
         if (kon[j]) {
 #pragma unroll
           for (int a = 0; a < 4; ++a) tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
-          gst_field<T, V4<T>>(sd.dX + 4 * (r + kz[j]), o);
+          gst_field_o<T, V4<T>>(sd.dX, (r + kz[j]) * (unsigned)sizeof(V4<T>), o);
         }
       }
       if (hx) {
@@ -828,22 +860,23 @@ k_stencilmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T*
     const int gy = item.y0 + w;
     const bool inner = w > 0 && w < NW - 1;  // rows whose potentials are computed here
     auto xw = [&](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
-    auto row_base = [&](int x, int y) { return ((long)x * n1 + y) * n2 + CPL * lane; };
+    // cell indices in 32 bits (the host keeps fields below 2^27 cells)
+    auto row_base = [&](int x, int y) { return ((unsigned)x * n1 + y) * n2 + CPL * lane; };
     auto load_full = [&](int x, int y, CellV<T>(&c)[CPL]) {
-      const long r = row_base(x, y);
+      const unsigned r = row_base(x, y);
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) c[j] = load_cell<T>(sd, r + j);
+      for (int j = 0; j < CPL; ++j) c[j] = load_cell32<T>(sd, r + j);
     };
     auto load_x = [&](int x, V4<T>(&v)[CPL]) {
-      const long r = row_base(x, gy);
+      const unsigned r = row_base(x, gy);
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) v[j] = ld4g<T>(sd.X + 4 * (r + j));
+      for (int j = 0; j < CPL; ++j) v[j] = gldo<V4<T>>(sd.X, (r + j) * (unsigned)sizeof(V4<T>));
     };
     // the row's psi at plane x, for the inner rows' potentials
     auto load_psi = [&](int x, T(&p)[CPL]) {
-      const long r = row_base(x, gy);
+      const unsigned r = row_base(x, gy);
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) p[j] = gld<T>(sd.psi + r + j);
+      for (int j = 0; j < CPL; ++j) p[j] = gldo<T>(sd.psi, (r + j) * (unsigned)sizeof(T));
     };
     int lo[4];
     stage_box4<T, NT>(sd, item, BOX, lo, tid);
@@ -909,14 +942,15 @@ k_stencilmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T*
         }
       } else {
         // boundary rows and planes: the potentials pass wrote them
-        const long r = row_base(xw(x + 1), gy);
+        const unsigned r = row_base(xw(x + 1), gy);
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
-          nx[j].mu = ld4g<T>(sd.mu + 4 * (r + j));
-          const V4<T> m = ld4g<T>(sd.mb + 4 * (r + j));
+          const unsigned o4 = (r + j) * (unsigned)sizeof(V4<T>);
+          nx[j].mu = gldo<V4<T>>(sd.mu, o4);
+          const V4<T> m = gldo<V4<T>>(sd.mb, o4);
 #pragma unroll
           for (int a = 0; a < 4; ++a) nx[j].M[a] = m.v[a];
-          nx[j].M[4] = gld<T>(sd.mb1 + r + j);
+          nx[j].M[4] = gldo<T>(sd.mb1, (r + j) * (unsigned)sizeof(T));
         }
         // the rows beside the tile, for the edge rows' y faces
         if (w == NW - 1)
@@ -988,7 +1022,7 @@ k_stencilmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T*
       T fxu[CPL][4];
 #pragma unroll
       for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], nx[j], inv_dx, fxu[j]);
-      const long r = row_base(x, gy);
+      const unsigned r = row_base(x, gy);
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
         // divergence in the reference order: x, y, z, each (F+ - F-) added in turn
@@ -1001,7 +1035,7 @@ k_stencilmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T*
           o.v[a] = acc * dscale;
           tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
         }
-        gst_field<T, V4<T>>(sd.dX + 4 * (r + j), o);
+        gst_field_o<T, V4<T>>(sd.dX, (r + j) * (unsigned)sizeof(V4<T>), o);
       }
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
