@@ -44,6 +44,40 @@ def test_served_equals_alone(precision):
         assert np.array_equal(X, r.X), f"request {r.rid} differs alone vs served"
 
 
+def test_served_equals_alone_workload_sizes():
+    """Requests of the serving workload's sizes (1D mw 50 and 200 x ppw 16, 2D 128^2,
+    256^2 and 512^2, 3D 64^3, 96^3 and 128^3, elasticity on), arriving while the
+    earlier ones run: every served FP32 result equals the request run alone, bit
+    for bit."""
+    from phasefield5d_amd.engine import ST_DONE
+    from phasefield5d_amd.problem import build_problem
+    from phasefield5d_amd.serve import Request, Server
+    from phasefield5d_amd.workloads import small_config
+    sizes = [(1, 800, 16), (2, 128, 16), (3, 64, 16), (2, 256, 16), (1, 3200, 16), (3, 96, 16), (2, 512, 16),
+             (3, 128, 16)]
+    reqs = []
+    for i, (dim, n, ppw) in enumerate(sizes):
+        pb = build_problem(small_config(dim, n, seed=40 + i, ppw=ppw))
+        steps = 120 + 20 * i
+        r = Request(i, pb.cfg, None, arrival=0.001 * i, max_steps=steps)
+        r.problem = pb
+        r.t_end = steps * pb.dt_cfl
+        reqs.append(r)
+        print(f"[serving] request {i}: {dim}D {pb.shape}, {steps} steps", flush=True)
+    srv = Server("fp32", poll_steps=5, setup_workers=2)
+    srv._setup = lambda req: req  # problems are prebuilt
+    done = srv.run(reqs)
+    srv.close()
+    assert len(done) == len(sizes)
+    overlapped = sum(1 for a in done for b in done if a is not b and a.admitted < b.finished and b.admitted < a.finished)
+    assert overlapped > 0, "no two requests were resident together"
+    for r in done:
+        assert r.status & ST_DONE
+        X, st = _alone(r.problem, r.t_end, "fp32")
+        assert st["steps"] == r.steps
+        assert np.array_equal(X, r.X), f"request {r.rid} {r.problem.shape} differs alone vs served"
+
+
 def test_admission_respects_budget():
     """With room for one request at a time, requests queue and still complete."""
     from phasefield5d_amd.serve import Request, Server, footprint_bytes
