@@ -16,6 +16,9 @@ Reported:
                groups that fit the last-level cache), and its ratio to fp64_plain
   fp32_developed_field  the FP32 step on a decomposed-looking field (smooth
                modes of amplitude 0.05 per component), as later in a run
+  fp32_fused_stencil  the same step with the fused marching stencil form
+               (k_stencilmarch, the potentials pass on the tiles' boundary cells
+               only; same bits), its in-step K1 time and 36 B/cell roofline
   roofline     the fused stencil stage K1 (interpolation + kappa lap + mu + face
                fluxes + divergence + block max) against its compulsory traffic,
                36 B/cell in FP32 (X 16 + psi 4 read, dX/dt 16 written): achieved =
@@ -240,7 +243,9 @@ def main():
     k2_ms = e32.time_kernel("k2prime", reps=10)
     dt_ms = e32.time_kernel("dt", reps=10)
     info = e32.launch_info()
-    split = e32.variants()[info["variant"]][0] == 0 if "variant" in info else False
+    # two-pass forms: by == 0 (the potentials pass on every cell) or -1 (the fused
+    # marching form, the potentials pass on the boundary cells)
+    split = e32.variants()[info["variant"]][0] <= 0 if "variant" in info else False
     kp_ms = kf_ms = None
     if split:
         kp_ms = e32.time_kernel("k1_potentials", reps=20)
@@ -291,6 +296,35 @@ def main():
                      "active": bool(ed.state(sd_)["status"] & 1)}
         ed.close()
         log("fp32 developed field", developed)
+
+    # the fused marching stencil form (k_stencilmarch; the potentials pass on the
+    # tiles' boundary rows and planes only), same FFT batching, runs of 8 planes:
+    # whole steps and the two K1 passes inside them
+    fused_leg = None
+    if not args.fp32_only:
+        ef = Engine("fp32")
+        sf = ef.add_problem(pb)
+        fv = next((i for i, v in enumerate(ef.variants()) if v[0] == -1), None)
+        if fv is not None:
+            ef.set_tiling(fv, 8)
+            ef.set_fft(info["fft_budget"] // 1024, info["fft_nt"], info["fft_parts"], info["fft_plane_nt"],
+                       wave=info["fft_wave"])
+            ef.step(args.warmup)
+            ef.prepare(args.steps)
+            ef.sync()
+            t0 = time.perf_counter()
+            ef.step(args.steps)
+            ef.sync()
+            elf = time.perf_counter() - t0
+            taken = ef.launch_info()["variant"] == fv
+            stf = ef.time_step_stages(50) if taken else None
+            fused_leg = {"variant": fv, "cx": 8, "taken": taken, "steps": args.steps,
+                         "ms_per_step": 1e3 * elf / args.steps, "cell_updates_per_s": ncell * args.steps / elf,
+                         "stage_ms_in_step": stf,
+                         "k1_ms_in_step": (stf["k1_potentials"] + stf["k1_flux"]) if stf else None}
+            # (the trajectory is the two-pass form's, bit for bit: tests/test_gpu_stencilmarch.py)
+        ef.close()
+        log("fp32 fused marching stencil", fused_leg)
 
     # batched FP32 at 128^3: eight simulations of the workload in one engine (the
     # engine steps them in groups that fit the last-level cache)
@@ -384,8 +418,9 @@ def main():
         with open(cpath) as f:
             counters = json.load(f).get("kernels", {})
 
-    def traffic_of(*stages):
-        vals = [counters[s_]["hbm_bytes"] for s_ in stages if s_ in counters and "hbm_bytes" in counters[s_]]
+    def traffic_of(*stages, table=None):
+        table = counters if table is None else table
+        vals = [table[s_]["hbm_bytes"] for s_ in stages if s_ in table and "hbm_bytes" in table[s_]]
         return sum(vals) if len(vals) == len(stages) else None
 
     def roofline(kname, bpc, kms, traffic=None, alone_ms=None):
@@ -401,13 +436,26 @@ def main():
             r["frac_alone"] = bpc * ncell / (alone_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
         return r
 
+    # the fused form's counters (its own profiled step)
+    fcounters = {}
+    fpath = os.path.join(ROOT, "profiles", "step_3d128_fp32_fused_stencil_counters.json")
+    if os.path.exists(fpath):
+        with open(fpath) as f:
+            fcounters = json.load(f).get("kernels", {})
+    fused_default = split and e32.variants()[info["variant"]][0] == -1
+    K1_TWO_PASS = ("K1 stencil stage: k_potentials (interpolation from the LDS node box + kappa lap + psi -> mu, "
+                   "Mob) + k_fluxmarch (face fluxes + divergence + block max; dt controller in its last block)")
+    K1_FUSED = ("K1 stencil stage, fused marching form: k_potentials on the tiles' boundary rows and planes + "
+                "k_stencilmarch (mu, Mob of the other cells, face fluxes, divergence, block max, dt controller)")
+    if fused_leg and fused_leg.get("k1_ms_in_step"):
+        fused_leg["roofline_k1"] = roofline(K1_FUSED, 36, fused_leg["k1_ms_in_step"],
+                                            traffic_of("potentials", "flux", table=fcounters))
     passes = None
     if split:
         kp_in = in_step["k1_potentials"] if in_step else kp_ms
         kf_in = in_step["k1_flux"] if in_step else kf_ms
-        roof = roofline("K1 stencil stage: k_potentials (interpolation from the LDS node box + kappa lap + psi "
-                        "-> mu, Mob) + k_fluxmarch (face fluxes + divergence + block max; dt controller in its "
-                        "last block)", 36, kp_in + kf_in, traffic_of("potentials", "flux"),
+        roof = roofline(K1_FUSED if fused_default else K1_TWO_PASS, 36, kp_in + kf_in,
+                        traffic_of("potentials", "flux", table=fcounters if fused_default else None),
                         alone_ms=(kp_ms + kf_ms) if kp_ms and kf_ms else None)
         passes = [roofline("k_potentials (its bytes include the mu, Mob it writes)", 56, kp_in,
                            traffic_of("potentials"), kp_ms),
@@ -468,6 +516,7 @@ def main():
         "mass_drift_fp32": drift32.tolist(),
         "error_vs_fp64_long": long_err,
         "fp32_developed_field": developed,
+        "fp32_fused_stencil": fused_leg,
         "stage_ms": {"k1": k1_ms, "k1_potentials": kp_ms, "k1_flux": kf_ms, "fft": fft_ms, "k2": k2_ms,
                      "dt": dt_ms},
         "stage_ms_in_step": in_step,
