@@ -552,8 +552,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
     // FP32: the tile's edge wavefronts load the halo row beyond it at the start
     // of each plane, so the load overlaps the exchange instead of stalling the
     // block after its first barrier (a tile of one row loads both in place).
-    // FP64 loads them in place: the extra row of registers made it spill
-    constexpr bool prefetch_halo = sizeof(T) == 4;
+    // FP64, and FP32 built for three waves per SIMD, load them in place: the
+    // extra row of registers made those builds spill
+    constexpr bool prefetch_halo = sizeof(T) == 4 && MINW < 3;
     const bool pre_up = prefetch_halo && hy && !ywrap && item.ty > 1 && w == item.ty - 1;
     const bool pre_dn = prefetch_halo && hy && !ywrap && item.ty > 1 && w == 0;
     for (int x = item.xa; x < item.xb; ++x) {
