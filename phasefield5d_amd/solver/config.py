@@ -1,9 +1,10 @@
 """Run configuration: dataclass + argparse parser.
 
-Keeps every flag of phasefield5d/solver/config.py:74-143 with the same names,
-defaults and parsing (``--x/--no_x`` boolean pairs, comma lists, ``direction``
-cast to int, exactly five elements).  Added flags (all defaulting to the
-reference behaviour):
+The command line of phasefield5d/solver/config.py:74-143 -- every flag name,
+default and group, the ``--x/--no_x`` boolean pairs, comma (or semicolon)
+separated number lists of a fixed length, ``direction`` truncated to integers,
+exactly five elements -- described here as one table (``FLAGS``) that the
+parser is built from.  Added flags (all defaulting to the reference behaviour):
 
 * ``--c11/--c12/--c44`` (GPa) override the hard-coded cubic constants;
 * ``--molar_volume`` (m^3/mol) overrides the linear-mixing molar volume;
@@ -14,33 +15,81 @@ reference behaviour):
 * ``--precision fp64|fp32`` selects the device arithmetic of the fused step;
 * ``--mode plain|fused`` selects the per-function drop-in loop or the
   device-resident fused step.
+
+Errors: a malformed number list is an ``argparse.ArgumentTypeError`` (argparse
+reports it and exits with status 2), a wrong element count a ``ValueError``.
 """
 import argparse
+import re
 from dataclasses import dataclass, field
 from typing import List, Optional
 
 import numpy as np
 
+NUM_ELEMENTS = 5  # Fe (dependent) and four independent components
 
-def parse_float_list(s, *, expected_len=None):
-    vals = [float(x) for x in s.replace(";", ",").split(",") if x.strip() != ""]
-    if expected_len is not None and len(vals) != expected_len:
-        raise argparse.ArgumentTypeError(f"Expected {expected_len} values, got {len(vals)}")
-    return vals
+_LIST_SEP = re.compile(r"[,;]")
 
 
-def composition_type(s, *, ncomp):
-    return np.array(parse_float_list(s, expected_len=ncomp), dtype=float)
+def number_list(count, dtype=float, shape=None):
+    """An argparse type: ``count`` numbers separated by commas or semicolons (blank
+    items skipped), as an array of ``dtype`` (values read as floats, then cast)."""
+    def parse(text):
+        items = [t.strip() for t in _LIST_SEP.split(text)]
+        items = [t for t in items if t]
+        if len(items) != count:
+            raise argparse.ArgumentTypeError(f"{text!r} holds {len(items)} numbers; {count} are needed")
+        arr = np.asarray([float(t) for t in items]).astype(dtype)
+        return arr.reshape(shape) if shape is not None else arr
+    return parse
 
 
-def add_bool_flag(parser, name, default=False, help_on="", help_off=""):
-    dest = name.replace("-", "_")
-    grp = parser.add_mutually_exclusive_group()
-    grp.add_argument(f"--{name}", dest=dest, action="store_true",
-                     help=help_on or f"Enable {name.replace('-', ' ')}.")
-    grp.add_argument(f"--no_{name}", dest=dest, action="store_false",
-                     help=help_off or f"Disable {name.replace('-', ' ')}.")
-    parser.set_defaults(**{dest: default})
+# One row per flag: (group, name, type, default, extra argparse keywords).  The
+# type BOOL makes the pair --name / --no_name with the given default.
+BOOL = "bool"
+FLAGS = [
+    ("thermodynamics", "elements", str, "Fe,Mn,Ni,Co,Cu", {}),
+    ("thermodynamics", "temperature", str, "873K", {}),
+    ("thermodynamics", "data_path", str, "", {}),
+    ("thermodynamics", "resolution", float, 0.01, {}),
+    ("thermodynamics", "system_dim", int, 1, {"choices": [1, 2, 3]}),
+    ("thermodynamics", "atomic_radius_tag", str, "senkov", {}),
+    ("composition", "initial_composition", number_list(4), "0.1,0.2,0.2,0.5", {}),
+    ("composition", "fluctuation", float, 1e-3, {}),
+    ("system size", "mw", int, 100, {}),
+    ("system size", "ppw", int, 16, {}),
+    ("energy gradient", "kappa_value", float, 5e-16, {}),
+    ("energy gradient", "kappa_i", number_list(4, shape=(1, 4)), "1,1,1,1", {}),
+    ("elasticity", "include_cubic_anisotropy", BOOL, True, {}),
+    ("elasticity", "direction", number_list(3, dtype=int), "1,0,0", {}),
+    ("elasticity", "c11", float, None, {"help": "override c11 [GPa]"}),
+    ("elasticity", "c12", float, None, {"help": "override c12 [GPa]"}),
+    ("elasticity", "c44", float, None, {"help": "override c44 [GPa]"}),
+    ("elasticity", "molar_volume", float, None, {"help": "override V_m [m^3/mol]"}),
+    ("time scale", "total_timesteps", int, 100_000, {}),
+    ("time scale", "steps_per_ctld", int, 1_000, {}),
+    ("time scale", "time_linear_end_multiplier", float, 100, {}),
+    ("adaptive step limits", "lower_limit", float, 1e-5, {}),
+    ("adaptive step limits", "upper_limit", float, 1e-3, {}),
+    ("adaptive step limits", "safety", float, 0.1, {}),
+    ("save logic", "early_stage_frames", int, 100, {}),
+    ("save logic", "late_burst_duration_ctld", int, 100, {}),
+    ("save logic", "late_burst_frames", int, 700, {}),
+    ("save logic", "late_tail_frames", int, 300, {}),
+    ("save logic", "save_outputs", BOOL, True, {}),
+    ("save logic", "results_root", str, "results", {}),
+    ("modifier", "same_initial_configurations", BOOL, False, {}),
+    ("modifier", "load_path", None, "", {}),
+    ("log name", "log_name", str, None, {}),
+    ("performance", "threads", int, 1, {}),
+    ("performance", "fft_workers", int, 1, {}),
+    ("device step", "precision", str, "fp64", {"choices": ["fp64", "fp32"]}),
+    ("device step", "mode", str, "plain", {"choices": ["plain", "fused"]}),
+    ("device step", "seed", int, None, {}),
+    ("device step", "synthetic_table", BOOL, False, {}),
+    ("device step", "table_seed", int, 0, {}),
+    ("device step", "table_diffusivity", float, 1e-20, {}),
+]
 
 
 @dataclass
@@ -91,94 +140,53 @@ class SimulationConfig:
 
 
 def build_parser(ncomp=4):
-    p = argparse.ArgumentParser(
+    """The argparse parser of FLAGS (``ncomp``: independent components, 4 here)."""
+    if ncomp != NUM_ELEMENTS - 1:
+        raise ValueError(f"this solver has {NUM_ELEMENTS - 1} independent components, not {ncomp}")
+    parser = argparse.ArgumentParser(
         prog="simulate_spinodal",
         description="Run multi-component Cahn-Hilliard / elastic simulation (MI355X step).",
         formatter_class=argparse.ArgumentDefaultsHelpFormatter)
-    th = p.add_argument_group("thermodynamics")
-    th.add_argument("--elements", type=str, default="Fe,Mn,Ni,Co,Cu")
-    th.add_argument("--temperature", type=str, default="873K")
-    th.add_argument("--data_path", type=str, default="")
-    th.add_argument("--resolution", type=float, default=0.01)
-    th.add_argument("--system_dim", type=int, choices=[1, 2, 3], default=1)
-    th.add_argument("--atomic_radius_tag", type=str, default="senkov")
-    co = p.add_argument_group("composition")
-    co.add_argument("--initial_composition", type=lambda s: composition_type(s, ncomp=4),
-                    default="0.1,0.2,0.2,0.5")
-    co.add_argument("--fluctuation", type=float, default=1e-3)
-    sz = p.add_argument_group("system size")
-    sz.add_argument("--mw", type=int, default=100)
-    sz.add_argument("--ppw", type=int, default=16)
-    ka = p.add_argument_group("energy gradient")
-    ka.add_argument("--kappa_value", type=float, default=5e-16)
-    ka.add_argument("--kappa_i", type=lambda s: np.array(parse_float_list(s, expected_len=4),
-                                                         dtype=float).reshape(1, 4), default="1,1,1,1")
-    el = p.add_argument_group("elasticity")
-    add_bool_flag(el, "include_cubic_anisotropy", default=True)
-    el.add_argument("--direction", type=lambda s: np.array(parse_float_list(s, expected_len=3), dtype=int),
-                    default="1,0,0")
-    el.add_argument("--c11", type=float, default=None, help="override c11 [GPa]")
-    el.add_argument("--c12", type=float, default=None, help="override c12 [GPa]")
-    el.add_argument("--c44", type=float, default=None, help="override c44 [GPa]")
-    el.add_argument("--molar_volume", type=float, default=None, help="override V_m [m^3/mol]")
-    ts = p.add_argument_group("time scale")
-    ts.add_argument("--total_timesteps", type=int, default=100_000)
-    ts.add_argument("--steps_per_ctld", type=int, default=1_000)
-    ts.add_argument("--time_linear_end_multiplier", type=float, default=100)
-    ad = p.add_argument_group("adaptive step limits")
-    ad.add_argument("--lower_limit", type=float, default=1e-5)
-    ad.add_argument("--upper_limit", type=float, default=1e-3)
-    ad.add_argument("--safety", type=float, default=0.1)
-    sv = p.add_argument_group("save logic")
-    sv.add_argument("--early_stage_frames", type=int, default=100)
-    sv.add_argument("--late_burst_duration_ctld", type=int, default=100)
-    sv.add_argument("--late_burst_frames", type=int, default=700)
-    sv.add_argument("--late_tail_frames", type=int, default=300)
-    add_bool_flag(sv, "save_outputs", default=True)
-    sv.add_argument("--results_root", type=str, default="results")
-    md = p.add_argument_group("modifier")
-    add_bool_flag(md, "same_initial_configurations", default=False)
-    md.add_argument("--load_path", default="")
-    lg = p.add_argument_group("log name")
-    lg.add_argument("--log_name", type=str, default=None)
-    pf = p.add_argument_group("performance")
-    pf.add_argument("--threads", type=int, default=1)
-    pf.add_argument("--fft_workers", type=int, default=1)
-    dv = p.add_argument_group("device step")
-    dv.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    dv.add_argument("--mode", choices=["plain", "fused"], default="plain")
-    dv.add_argument("--seed", type=int, default=None)
-    add_bool_flag(dv, "synthetic_table", default=False)
-    dv.add_argument("--table_seed", type=int, default=0)
-    dv.add_argument("--table_diffusivity", type=float, default=1e-20)
-    return p
+    groups = {}
+    for title, name, kind, default, extra in FLAGS:
+        if title not in groups:
+            groups[title] = parser.add_argument_group(title)
+        g = groups[title]
+        if kind is BOOL:
+            pair = g.add_mutually_exclusive_group()
+            words = name.replace("_", " ")
+            pair.add_argument(f"--{name}", dest=name, action="store_true", help=f"turn {words} on")
+            pair.add_argument(f"--no_{name}", dest=name, action="store_false", help=f"turn {words} off")
+            parser.set_defaults(**{name: default})
+        else:
+            kw = dict(extra, default=default)
+            if kind is not None:
+                kw["type"] = kind
+            g.add_argument(f"--{name}", **kw)
+    return parser
 
 
 def parse_args_to_config(argv=None):
-    a = build_parser(ncomp=4).parse_args(argv)
-    elements = [e.strip() for e in a.elements.split(",") if e.strip()]
-    if len(elements) != 5:
-        raise ValueError(f"--elements must list exactly 5 elements (got {len(elements)}: {elements}). "
-                         "phasefield5d is a 5-component solver; the number of components is fixed.")
-    return SimulationConfig(
-        elements=elements, temperature=str(a.temperature), data_path=str(a.data_path),
-        resolution=float(a.resolution), system_dim=int(a.system_dim),
-        atomic_radius_tag=str(a.atomic_radius_tag),
-        initial_composition=np.asarray(a.initial_composition, dtype=float),
-        fluctuation=float(a.fluctuation), mw=int(a.mw), ppw=int(a.ppw),
-        kappa_value=float(a.kappa_value), kappa_i=np.asarray(a.kappa_i, dtype=float),
-        include_cubic_anisotropy=bool(a.include_cubic_anisotropy),
-        direction=np.asarray(a.direction, dtype=int), total_timesteps=int(a.total_timesteps),
-        steps_per_ctld=int(a.steps_per_ctld),
-        time_linear_end_multiplier=float(a.time_linear_end_multiplier),
-        lower_limit=float(a.lower_limit), upper_limit=float(a.upper_limit), safety=float(a.safety),
-        early_stage_frames=int(a.early_stage_frames),
-        late_burst_duration_ctld=int(a.late_burst_duration_ctld),
-        late_burst_frames=int(a.late_burst_frames), late_tail_frames=int(a.late_tail_frames),
-        same_initial_configurations=bool(a.same_initial_configurations),
-        load_path=str(a.load_path), log_name=a.log_name or "", threads=int(a.threads),
-        fft_workers=int(a.fft_workers), c11=a.c11, c12=a.c12, c44=a.c44,
-        molar_volume=a.molar_volume, seed=a.seed, synthetic_table=bool(a.synthetic_table),
-        table_seed=int(a.table_seed), table_diffusivity=float(a.table_diffusivity),
-        precision=a.precision, mode=a.mode, results_root=a.results_root,
-        save_outputs=bool(a.save_outputs))
+    """The SimulationConfig of a command line (sys.argv when argv is None)."""
+    ns = build_parser().parse_args(argv)
+    elements = [e for e in (s.strip() for s in ns.elements.split(",")) if e]
+    if len(elements) != NUM_ELEMENTS:
+        raise ValueError(f"--elements names {len(elements)} elements ({elements}); the solver is built for "
+                         f"exactly {NUM_ELEMENTS} (the first one dependent)")
+    values = vars(ns)
+    values["elements"] = elements
+    casts = {"temperature": str, "data_path": str, "resolution": float, "system_dim": int,
+             "atomic_radius_tag": str, "fluctuation": float, "mw": int, "ppw": int, "kappa_value": float,
+             "include_cubic_anisotropy": bool, "total_timesteps": int, "steps_per_ctld": int,
+             "time_linear_end_multiplier": float, "lower_limit": float, "upper_limit": float, "safety": float,
+             "early_stage_frames": int, "late_burst_duration_ctld": int, "late_burst_frames": int,
+             "late_tail_frames": int, "same_initial_configurations": bool, "load_path": str, "threads": int,
+             "fft_workers": int, "synthetic_table": bool, "table_seed": int, "table_diffusivity": float,
+             "save_outputs": bool}
+    for key, cast in casts.items():
+        values[key] = cast(values[key])
+    values["initial_composition"] = np.asarray(values["initial_composition"], dtype=float)
+    values["kappa_i"] = np.asarray(values["kappa_i"], dtype=float)
+    values["direction"] = np.asarray(values["direction"], dtype=int)
+    values["log_name"] = values["log_name"] or ""
+    return SimulationConfig(**values)
