@@ -29,6 +29,8 @@ def main(argv=None):
     ap.add_argument("--wave", type=int, default=None, help="FFT wave-resident mode (0 = off, 1..3)")
     ap.add_argument("--tuned", action="store_true", help="use the autotuner (cached result if present)")
     ap.add_argument("--sims", type=int, default=1, help="copies of the workload in the engine")
+    ap.add_argument("--developed", type=float, default=0.0, metavar="AMP",
+                    help="start from a developed field: smooth modes of amplitude AMP per component")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="end with a 256 MiB device-to-device copy (to scale FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args(argv)
@@ -40,6 +42,10 @@ def main(argv=None):
 
     _lib.require_gpu()
     pb = build_problem(workload_config(a.workload, seed=0))
+    if a.developed > 0:
+        import dataclasses
+        from .workloads import developed_field
+        pb = dataclasses.replace(pb, X_init=developed_field(pb, amp=a.developed, seed=4))
     eng = Engine(a.precision)
     for _ in range(a.sims):
         eng.add_problem(pb)
