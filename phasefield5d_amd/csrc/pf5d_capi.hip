@@ -195,7 +195,9 @@ static void upload_table(pf5d_table* t, const double* grid, int nprops) {
   // of the linear region (sum X <= 1 - 4h - 1e-9)
   t->cube_s = t->num - 5;
   t->cubes = nullptr;
-  const long ncubes = simplex_count4(t->cube_s);
+  // (cube_rank's 32-bit arithmetic holds up to CUBE_MAX_S; a finer table, whose
+  // cube table would pass half a terabyte, interpolates from the dense rows)
+  const long ncubes = t->cube_s <= CUBE_MAX_S ? simplex_count4(t->cube_s) : 0;
   if (ncubes > 0) {
     t->cubes = dmalloc((size_t)ncubes * CUBE_VALUES * sizeof(T));
     const long m = t->num - 1, npos = m * m * m * m;

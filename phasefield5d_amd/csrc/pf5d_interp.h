@@ -61,6 +61,21 @@ PF5D_HD long simplex_rank4(int i0, int i1, int i2, int i3, int S) {
   const long R1 = S - i0, R2 = R1 - i1;
   return (f4(S) - f4(R1)) + (f3(R1) - f3(R1 - i1)) + (f2(R2) - f2(R2 - i2)) + i3;
 }
+// simplex_rank4 in 32-bit arithmetic, for S <= CUBE_MAX_S: the binomials
+// C(r + k, k) are built as C(r+2,2) = (r+1)(r+2)/2, C(r+3,3) = C(r+2,2)(r+3)/3,
+// C(r+4,4) = C(r+3,3)(r+4)/4, and for r <= 390 every factor is below 2^24 and
+// every product below 2^32, so each multiply is a 24-bit one (the masks state
+// the range to the compiler and change no value) and each division exact.  The
+// interpolation takes the rank per point; 64-bit products and divisions cost it
+// several times the instructions.
+constexpr int CUBE_MAX_S = 390;
+PF5D_DEV unsigned binom2(unsigned r) { return ((r & 0xFFFFu) + 1) * ((r & 0xFFFFu) + 2) >> 1; }
+PF5D_DEV unsigned binom3(unsigned r) { return (binom2(r) & 0xFFFFFFu) * ((r & 0xFFFFu) + 3) / 3; }
+PF5D_DEV unsigned binom4(unsigned r) { return (binom3(r) & 0xFFFFFFu) * ((r & 0xFFFFu) + 4) >> 2; }
+PF5D_DEV unsigned cube_rank(int i0, int i1, int i2, int i3, int S) {
+  const unsigned R1 = S - i0, R2 = R1 - i1;
+  return (binom4(S) - binom4(R1)) + (binom3(R1) - binom3(R1 - i1)) + (binom2(R2) - binom2(R2 - i2)) + i3;
+}
 PF5D_HD long simplex_count4(int S) { return S < 0 ? 0 : (long)(S + 1) * (S + 2) * (S + 3) * (S + 4) / 24; }
 
 template <typename T>
@@ -234,7 +249,7 @@ PF5D_DEV void interp_gather(const TableView<T>& tv, int i0, int i1, int i2, int 
                             T P[9]) {
   constexpr int NP = TablePad<T>::value;
   if (tv.cubes && i0 + i1 + i2 + i3 <= tv.cube_s) {
-    interp_cube<T>(tv.cubes + simplex_rank4(i0, i1, i2, i3, tv.cube_s) * CUBE_VALUES, t0, t1, t2, t3, P);
+    interp_cube<T>(tv.cubes + (size_t)cube_rank(i0, i1, i2, i3, tv.cube_s) * CUBE_VALUES, t0, t1, t2, t3, P);
   } else {
     const int num = tv.num;
     const T* base = tv.data + ((((long)i0 * num + i1) * num + i2) * num + i3) * NP;
@@ -346,8 +361,16 @@ PF5D_DEV void interp9_box4(const TableView<T>& tv, const T* box, const int lo[4]
   axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
   axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
   const int d0 = i0 - lo[0], d1 = i1 - lo[1], d2 = i2 - lo[2], d3 = i3 - lo[3];
-  if ((unsigned)d0 <= BOX4 - 2 && (unsigned)d1 <= BOX4 - 2 && (unsigned)d2 <= BOX4 - 2 &&
-      (unsigned)d3 <= BOX4 - 2) {
+  const bool inbox = (unsigned)d0 <= BOX4 - 2 && (unsigned)d1 <= BOX4 - 2 && (unsigned)d2 <= BOX4 - 2 &&
+                     (unsigned)d3 <= BOX4 - 2;
+  // A wavefront whose points all lie in the box reads it; one that mixes points
+  // inside and outside (a developed field) sends every point the cube table
+  // covers to the table, so it runs the 144 multiply-adds once rather than once
+  // per source under opposite masks.  Box and cube table hold the same node
+  // values and the sums run in the same order: the result does not depend on
+  // the source.
+  const bool covered = tv.cubes && i0 + i1 + i2 + i3 <= tv.cube_s;
+  if (inbox && (__all(inbox) || !covered)) {
     using L = BoxLayout<T>;
     constexpr int VN = Vec16<T>::n;
     static_assert(L::S3 * VN == NP, "axis-3 stride is one padded row");
