@@ -734,7 +734,7 @@ constexpr int INIT_NT = 256;
 // block leaves Neumaier partial sums of the FP64 field (sum, compensation per
 // component) in part[8 * block ...] for the FP64 mean.
 template <typename T>
-__global__ void __launch_bounds__(INIT_NT) k_init_field(T* __restrict__ X, T* __restrict__ Xlo, long ncell,
+__global__ void __launch_bounds__(INIT_NT) k_init_field(T* __restrict__ X, Rem4* __restrict__ Xlo, long ncell,
                                                         double x00, double x01, double x02, double x03, double f,
                                                         unsigned long long seed, double* __restrict__ part) {
   const double X0[4] = {x00, x01, x02, x03};
@@ -747,18 +747,19 @@ __global__ void __launch_bounds__(INIT_NT) k_init_field(T* __restrict__ X, T* __
       m += n[k];
     }
     m /= 5.0;
-    V4<T> hi, lo;
+    V4<T> hi;
+    Rem4 lo;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const double x = X0[a] + f * (n[a + 1] - m);
       hi.v[a] = (T)x;
-      lo.v[a] = (T)(x - (double)hi.v[a]);
+      lo.v[a] = rem_encode(x, (float)hi.v[a]);
       const double t = sum[a] + x;
       cmp[a] += (fabs(sum[a]) >= fabs(x)) ? ((sum[a] - t) + x) : ((x - t) + sum[a]);
       sum[a] = t;
     }
     st4g<T>(X + 4 * c, hi);
-    if (Xlo) st4g<T>(Xlo + 4 * c, lo);
+    if (Xlo) gst<Rem4>(Xlo + c, lo);
   }
   __shared__ double red[INIT_NT / 64][8];
   double v[8] = {sum[0], cmp[0], sum[1], cmp[1], sum[2], cmp[2], sum[3], cmp[3]};
@@ -960,7 +961,9 @@ struct Engine : EngineBase {
     bool used = false;
     pf5d_sim_desc d{};
     long ncell = 0;
-    T *X = nullptr, *Xlo = nullptr, *dX = nullptr, *psi = nullptr, *s = nullptr, *bhat = nullptr;
+    T* X = nullptr;
+    Rem4* Xlo = nullptr;
+    T *dX = nullptr, *psi = nullptr, *s = nullptr, *bhat = nullptr;
     T *mu = nullptr, *mb = nullptr, *mb1 = nullptr;  // two-pass stencil intermediates
     cx<T>* C = nullptr;
     const pf5d_table* tab = nullptr;
@@ -1034,9 +1037,9 @@ struct Engine : EngineBase {
     }
   };
   std::vector<std::unique_ptr<Group>> groups;
-  // per-step working set, bytes per cell: X (+ remainder), dX, mu, Mob, s, psi
-  // and the spectrum; a group holds at most GROUP_BYTES of it
-  static constexpr size_t STEP_BYTES = sizeof(T) == 4 ? 100 : 170;
+  // per-step working set, bytes per cell: X (+ 16-bit remainder), dX, mu, Mob, s,
+  // psi and the spectrum; a group holds at most GROUP_BYTES of it
+  static constexpr size_t STEP_BYTES = sizeof(T) == 4 ? 92 : 170;
   static constexpr size_t GROUP_BYTES = size_t(192) << 20;
 
   Engine(hipStream_t s) {
@@ -1213,8 +1216,8 @@ struct Engine : EngineBase {
       s.mb1 = (T*)dmalloc(ncell * sizeof(T));
       s.bytes = ncell * 17 * sizeof(T);
       if (sizeof(T) == 4) {
-        s.Xlo = (T*)dmalloc(ncell * 4 * sizeof(T));
-        s.bytes += ncell * 4 * sizeof(T);
+        s.Xlo = (Rem4*)dmalloc(ncell * sizeof(Rem4));
+        s.bytes += ncell * sizeof(Rem4);
       }
       if (d->elastic) {
         s.psi = (T*)dmalloc(ncell * sizeof(T));
@@ -1240,7 +1243,7 @@ struct Engine : EngineBase {
     *slot = id;
     dirty = true;
   }
-  // FP32 mode keeps X as the nearest FP32 value plus an FP32 remainder
+  // FP32 mode keeps X as the nearest FP32 value plus a 16-bit remainder
   void upload_X(Sim& s, const double* Xh) {
     // FP64 mean of the field (pairwise over cells is unnecessary at this size: Neumaier)
     for (int a = 0; a < 4; ++a) {
@@ -1252,13 +1255,14 @@ struct Engine : EngineBase {
       }
       s.mean0[a] = (sum + c) / (double)s.ncell;
     }
-    std::vector<T> h(s.ncell * 4), lo(s.Xlo ? s.ncell * 4 : 0);
+    std::vector<T> h(s.ncell * 4);
+    std::vector<short> lo(s.Xlo ? s.ncell * 4 : 0);
     for (long i = 0; i < s.ncell * 4; ++i) {
       h[i] = (T)Xh[i];
-      if (s.Xlo) lo[i] = (T)(Xh[i] - (double)h[i]);
+      if (s.Xlo) lo[i] = rem_encode(Xh[i], (float)h[i]);
     }
     HIPCHK(hipMemcpy(s.X, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-    if (s.Xlo) HIPCHK(hipMemcpy(s.Xlo, lo.data(), lo.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (s.Xlo) HIPCHK(hipMemcpy(s.Xlo, lo.data(), lo.size() * sizeof(short), hipMemcpyHostToDevice));
   }
   Sim& at(int slot) {
     require(slot >= 0 && slot < (int)sims.size() && sims[slot].used, "invalid slot " + std::to_string(slot));
@@ -1786,11 +1790,12 @@ struct Engine : EngineBase {
     const T* src = which == 0 ? s.X : which == 1 ? s.dX : s.psi;
     require(src != nullptr, "field not present");
     const long n = (which == 2) ? s.ncell : s.ncell * 4;
-    std::vector<T> h(n), lo((which == 0 && s.Xlo) ? n : 0);
+    std::vector<T> h(n);
+    std::vector<short> lo((which == 0 && s.Xlo) ? n : 0);
     HIPCHK(hipMemcpyAsync(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
-    if (!lo.empty()) HIPCHK(hipMemcpyAsync(lo.data(), s.Xlo, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    if (!lo.empty()) HIPCHK(hipMemcpyAsync(lo.data(), s.Xlo, n * sizeof(short), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    for (long i = 0; i < n; ++i) out[i] = lo.empty() ? (double)h[i] : (double)h[i] + (double)lo[i];
+    for (long i = 0; i < n; ++i) out[i] = lo.empty() ? (double)h[i] : rem_decode((float)h[i], lo[i]);
   }
   void set_field(int slot, const double* X) override {
     Sim& s = at(slot);

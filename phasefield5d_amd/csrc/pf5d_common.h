@@ -116,6 +116,29 @@ PF5D_DEV void gst_field_o(void* base, unsigned off, const V& v) {
   gst_field<T, V>(static_cast<char*>(base) + off, v);
 }
 
+// FP32 state remainder in 16 bits.  The FP64 state x is held as the nearest
+// float hi and lo = rint((x - hi) / q), q = ulp(hi) / 2^15 (for zero and
+// subnormal hi the ulp of the smallest normal float): |x - hi| <= ulp(hi) / 2,
+// so |lo| <= 2^14, and hi + lo q is within q / 2 of x -- a 39-bit significand
+// whose exact FP64 sum needs no rounding.  q is a power of two (2^(b - 165) for
+// the biased exponent b of hi), so every scaling is exact.  Infinite or NaN hi
+// keeps lo = 0.
+struct alignas(8) Rem4 {
+  short v[4];
+};
+PF5D_HD int rem_exp(float hi) {
+  unsigned bits;
+  __builtin_memcpy(&bits, &hi, 4);
+  const int b = (int)((bits >> 23) & 0xFFu);
+  return b < 1 ? 1 : b;
+}
+PF5D_HD short rem_encode(double x, float hi) {
+  const int b = rem_exp(hi);
+  if (b == 255) return 0;
+  return (short)(int)__builtin_rint((x - (double)hi) * __builtin_ldexp(1.0, 165 - b));
+}
+PF5D_HD double rem_decode(float hi, short lo) { return (double)hi + (double)lo * __builtin_ldexp(1.0, rem_exp(hi) - 165); }
+
 template <typename T>
 PF5D_DEV V4<T> ld4g(const T* p) { return gld<V4<T>>(p); }
 template <typename T>

@@ -39,7 +39,7 @@ struct SimDev {
   int fresh;          // the field was just set: a prime launch writes its misfit s
   TableView<T> tab;
   T* X;
-  T* Xlo;             // FP32 mode: rounding remainder, the state is X + Xlo (null in FP64 mode)
+  Rem4* Xlo;          // FP32 mode: 16-bit rounding remainder of each value (rem_decode; null in FP64 mode)
   T* dX;
   const T* psi;
   T* s;
@@ -550,11 +550,11 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
 #pragma unroll 2
     for (long c = item.c0 + threadIdx.x; c < item.c1; c += NT) {
       V4<T> x = ld4g<T>(sd.X + 4 * c);
-      double xd[4];  // the state in FP64 (FP32 mode: X + Xlo)
+      double xd[4];  // the state in FP64 (FP32 mode: X with its remainder)
       if (sd.Xlo) {
-        const V4<T> lo = ld4g<T>(sd.Xlo + 4 * c);
+        const Rem4 lo = gld<Rem4>(sd.Xlo + c);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a] + (double)lo.v[a];
+        for (int a = 0; a < 4; ++a) xd[a] = rem_decode((float)x.v[a], lo.v[a]);
       } else {
 #pragma unroll
         for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
@@ -575,7 +575,7 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
           for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
           st4g<T>(sd.X + 4 * c, x);
         } else {
-          // FP32 state with an FP32 remainder: update and clamp in FP64, then split
+          // FP32 state with a 16-bit remainder: update and clamp in FP64, then split
           // into the nearest FP32 value and what it leaves out, so increments below
           // half an FP32 ulp accumulate instead of being rounded away
 #pragma unroll
@@ -586,14 +586,14 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
 #pragma unroll
             for (int a = 0; a < 4; ++a) xd[a] = xd[a] * inv;
           }
-          V4<T> lo;
+          Rem4 lo;
 #pragma unroll
           for (int a = 0; a < 4; ++a) {
             x.v[a] = (T)xd[a];
-            lo.v[a] = (T)(xd[a] - (double)x.v[a]);
+            lo.v[a] = rem_encode(xd[a], (float)x.v[a]);
           }
           st4g<T>(sd.X + 4 * c, x);
-          st4g<T>(sd.Xlo + 4 * c, lo);
+          gst<Rem4>(sd.Xlo + c, lo);
         }
       }
       // the misfit s: from the updated state in a step; a prime launch writes it
