@@ -243,19 +243,17 @@ def main():
     k2_ms = e32.time_kernel("k2prime", reps=10)
     dt_ms = e32.time_kernel("dt", reps=10)
     info = e32.launch_info()
-    # two-pass forms: by == 0 (the potentials pass on every cell) or -1 (the fused
-    # marching form, the potentials pass on the boundary cells)
-    split = e32.variants()[info["variant"]][0] <= 0 if "variant" in info else False
-    kp_ms = kf_ms = None
-    if split:
-        kp_ms = e32.time_kernel("k1_potentials", reps=20)
-        kf_ms = e32.time_kernel("k1_flux", reps=20)
+    # every stencil form has a potentials and a flux pass: by == 0 (the potentials
+    # pass on every cell) or -1 (the fused marching form, the potentials pass on
+    # the boundary cells)
+    kp_ms = e32.time_kernel("k1_potentials", reps=20)
+    kf_ms = e32.time_kernel("k1_flux", reps=20)
     log(f"fp32: {v32:.4g} cell-updates/s, {1e3 * el32 / args.steps:.3f} ms/step; K1 {k1_ms:.3f} ms, FFT {fft_ms:.3f} ms",
         info)
     # per-stage durations inside whole steps (HIP events between the stages of steps
     # launched one by one), on a second engine holding the same workload, so the
     # headline simulation keeps its step count
-    march = split and e32.variants()[info["variant"]][3] > 0
+    march = e32.variants()[info["variant"]][3] > 0
     in_step = in_step_sep = None
     if march:
         et = Engine("fp32")
@@ -292,7 +290,7 @@ def main():
         ed.sync()
         eld = time.perf_counter() - t0
         developed = {"amplitude": 0.05, "steps": 50, "ms_per_step": 1e3 * eld / 50, "cell_updates_per_s": ncell * 50 / eld,
-                     "k1_potentials_ms": ed.time_kernel("k1_potentials", reps=10) if split else None,
+                     "k1_potentials_ms": ed.time_kernel("k1_potentials", reps=10),
                      "active": bool(ed.state(sd_)["status"] & 1)}
         ed.close()
         log("fp32 developed field", developed)
@@ -442,7 +440,7 @@ def main():
     if os.path.exists(fpath):
         with open(fpath) as f:
             fcounters = json.load(f).get("kernels", {})
-    fused_default = split and e32.variants()[info["variant"]][0] == -1
+    fused_default = e32.variants()[info["variant"]][0] == -1
     K1_TWO_PASS = ("K1 stencil stage: k_potentials (interpolation from the LDS node box + kappa lap + psi -> mu, "
                    "Mob) + k_fluxmarch (face fluxes + divergence + block max; dt controller in its last block)")
     K1_FUSED = ("K1 stencil stage, fused marching form: k_potentials on the tiles' boundary rows and planes + "
@@ -450,20 +448,14 @@ def main():
     if fused_leg and fused_leg.get("k1_ms_in_step"):
         fused_leg["roofline_k1"] = roofline(K1_FUSED, 36, fused_leg["k1_ms_in_step"],
                                             traffic_of("potentials", "flux", table=fcounters))
-    passes = None
-    if split:
-        kp_in = in_step["k1_potentials"] if in_step else kp_ms
-        kf_in = in_step["k1_flux"] if in_step else kf_ms
-        roof = roofline(K1_FUSED if fused_default else K1_TWO_PASS, 36, kp_in + kf_in,
-                        traffic_of("potentials", "flux", table=fcounters if fused_default else None),
-                        alone_ms=(kp_ms + kf_ms) if kp_ms and kf_ms else None)
-        passes = [roofline("k_potentials (its bytes include the mu, Mob it writes)", 56, kp_in,
-                           traffic_of("potentials"), kp_ms),
-                  roofline("k_fluxmarch (its bytes include the mu, Mob it reads)", 68, kf_in, traffic_of("flux"),
-                           kf_ms)]
-    else:
-        roof = roofline("K1 stencil stage: k_stencil (fused interpolation + kappa lap + mu + face flux + "
-                        "divergence)", 36, k1_ms)
+    kp_in = in_step["k1_potentials"] if in_step else kp_ms
+    kf_in = in_step["k1_flux"] if in_step else kf_ms
+    roof = roofline(K1_FUSED if fused_default else K1_TWO_PASS, 36, kp_in + kf_in,
+                    traffic_of("potentials", "flux", table=fcounters if fused_default else None),
+                    alone_ms=kp_ms + kf_ms)
+    passes = [roofline("k_potentials (its bytes include the mu, Mob it writes)", 56, kp_in,
+                       traffic_of("potentials"), kp_ms),
+              roofline("k_fluxmarch (its bytes include the mu, Mob it reads)", 68, kf_in, traffic_of("flux"), kf_ms)]
     step_ms = in_step["step"] if in_step else 1e3 * el32 / args.steps
     roof_step = roofline("whole step (FFT pair, K1, dt, update)", 96, 1e3 * el32 / args.steps,
                          traffic_of("fft_fwd", "fft_x", "fft_inv", "potentials", "flux", "update"))

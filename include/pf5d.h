@@ -143,7 +143,9 @@ typedef struct pf5d_sim_desc {
 /* stream: NULL -> the engine creates its own non-blocking stream */
 int pf5d_engine_create(int prec, void* stream, pf5d_engine** out);
 int pf5d_engine_destroy(pf5d_engine* e);
-/* X_host: (ncell,4) float64; bhat_host as for pf5d_elastic_create; returns slot */
+/* X_host: (ncell,4) float64; bhat_host as for pf5d_elastic_create; returns slot.
+ * A simulation holds at most 2^28 cells in FP32 and 2^27 in FP64 (PF5D_EINVAL
+ * beyond: the step kernels use 32-bit offsets into its fields). */
 int pf5d_engine_add(pf5d_engine* e, const pf5d_sim_desc* d, const double* X_host, const double* bhat_host,
                     const pf5d_table* tab, int* slot);
 /* pf5d_engine_add with the initial field drawn on the device instead of by
@@ -156,9 +158,9 @@ int pf5d_engine_add_generated(pf5d_engine* e, const pf5d_sim_desc* d, unsigned l
                               const double* bhat_host, const pf5d_table* tab, int* slot);
 int pf5d_engine_remove(pf5d_engine* e, int slot);
 /* K1 tile: variant index (pf5d_engine_num_variants), x chunk (3D), per dimension.
- * info (5*n ints, may be NULL), per variant: by, bz, nt (by == 0: two-pass form;
- * by == -1: fused marching form, whose potentials pass covers only the marching
- * tiles' boundary rows and planes), then the flux pass of the two-pass form:
+ * info (5*n ints, may be NULL), per variant: by, bz, nt of the potentials pass's
+ * tiles (by == 0: two-pass form; by == -1: fused marching form, whose potentials
+ * pass covers only the marching tiles' boundary rows and planes), then the flux pass:
  * cells per lane (0: per-plane tiles, > 0: marching along axis 0 in x chunks)
  * and its block size.  A variant that does not fit the simulations held at the
  * next step (the fused form: 3D, rows of 128 cells, axis 1 a multiple of 8,
@@ -204,7 +206,7 @@ void* pf5d_engine_stream(pf5d_engine* e);
 /* number of active slots and bytes of device memory held */
 int pf5d_engine_stats(pf5d_engine* e, long* active, long* bytes);
 /* launch configuration of the current work lists, for profiling reports: [K1 items, K2 items,
- * FFT items, K1 grid, K1 LDS bytes, stencil variant, x chunk, FFT LDS budget bytes, FFT block
+ * FFT items, K1 grid, 0 (dynamic K1 LDS: none), stencil variant, x chunk, FFT LDS budget bytes, FFT block
  * size, flux-pass grid, FFT plane parts in use (0: line passes only), FFT plane block size,
  * mask of the FFT passes running wave-resident (pf5d_engine_set_fft_wave)] */
 int pf5d_engine_launch_info(pf5d_engine* e, long out[13]);
@@ -213,9 +215,8 @@ int pf5d_engine_launch_info(pf5d_engine* e, long out[13]);
 int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups);
 
 /* average device time of one launch of a stage (HIP events on the engine stream):
- * which = 0 stencil stage K1 (either form), 1 the FFT passes, 2 dt controller, 3 update
- * kernel in its statistics-only mode, 4 / 5 the potentials / flux pass of the two-pass
- * stencil form (when selected).  The timed launches rewrite the dX/dt and psi fields
+ * which = 0 stencil stage K1 (both of its passes), 1 the FFT passes, 2 dt controller, 3
+ * update kernel in its statistics-only mode, 4 / 5 the potentials / flux pass of K1.  The timed launches rewrite the dX/dt and psi fields
  * from the current state; the call then restores what the next step reads (the
  * max|dX/dt| accumulator, the FP64 statistics), so the trajectory is unchanged. */
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms);

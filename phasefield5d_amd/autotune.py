@@ -1,10 +1,9 @@
 """Autotuner for the stencil stage and the FFT passes.
 
 For an engine holding its simulations, every compiled K1 variant -- the
-single-pass fused tilings (BY x BZ tile, threads, register budget) combined
-with x-chunk lengths for 3D marching, and the two-pass forms (reported as
-tile 0 x 64, with the flux pass's cells per lane and block size; the marching
-flux pass takes x-chunk lengths too) -- is timed with HIP events on the engine
+two-pass forms (potentials tiles, with the flux pass's cells per lane and
+block size; the marching flux pass combined with x-chunk lengths for 3D) and
+the fused marching form (also with x-chunk lengths) -- is timed with HIP events on the engine
 stream (``Engine.time_kernel``); the fastest is kept.  Then the FFT: the line
 passes' batching (LDS per item, block size), the plane passes of 3D fields
 with 128 x 128 planes (parts per plane, block size), and the line batching
@@ -64,9 +63,8 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
     has3d = any(len(pb.shape) == 3 for pb, _ in engine.slots.values())
     results = []
     for v, (by, bz, nt, fcpl, fnt) in enumerate(engine.variants()):
-        # the x-chunk length applies to the single-pass form (by > 0) and to the
-        # marching flux pass of the two-pass form (fcpl > 0)
-        marching = by > 0 or fcpl > 0
+        # the x-chunk length applies to the marching passes (fcpl > 0)
+        marching = fcpl > 0
         for cx in (chunks if has3d and marching else chunks[:1]):
             try:
                 engine.set_tiling(v, cx)

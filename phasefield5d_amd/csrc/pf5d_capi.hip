@@ -868,9 +868,7 @@ struct pf5d_engine {
 
 template <typename T>
 struct K1Variant {
-  int by, bz, nt;  // by == 0: two-pass form, tiles of nt cells in rows of up to bz
-  size_t lds;
-  void (*fn)(const SimDev<T>*, SimState*, T*, const K1Item*, int);
+  int by, bz, nt;  // potentials tiles of nt cells in rows of up to bz (by: 0)
   void (*fpot)(const SimDev<T>*, const SimState*, const K1Item*, int);
   void (*fflux)(const SimDev<T>*, const SimState*, T*, const K1Item*, int);
   // flux pass of the two-pass form: fcpl == 0 runs on the potentials pass's
@@ -887,19 +885,14 @@ struct K1Variant {
   bool fused = false;
   // potentials pass with the nearest-row search for tables with missing rows
   void (*fpot_wide)(const SimDev<T>*, const SimState*, const K1Item*, int) = nullptr;
-  bool split() const { return by == 0; }
 };
 
-template <typename T, int BY, int BZ, int NT, int MINW>
-static K1Variant<T> make_variant() {
-  return {BY, BZ, NT, K1Smem<T, BY, BZ>::bytes, &k_stencil<T, BY, BZ, NT, MINW>, nullptr, nullptr};
-}
 // MINW: register budget of the potentials pass; LOADS: how the flux pass
 // fetches the neighbour cells (0 per axis, 1 all up front, 2 up front with the
 // z neighbours from adjacent lanes); BZ: row length of a tile
 template <typename T, int NT, int MINW, int LOADS, int BZ = 64>
 static K1Variant<T> make_split_variant() {
-  K1Variant<T> v{0, BZ, NT, 0, nullptr, &k_potentials<T, NT, MINW, false>, &k_fluxdiv<T, NT, LOADS>, 0, NT};
+  K1Variant<T> v{0, BZ, NT, &k_potentials<T, NT, MINW, false>, &k_fluxdiv<T, NT, LOADS>, 0, NT};
   v.fpot_wide = &k_potentials<T, NT, MINW, true>;
   return v;
 }
@@ -909,8 +902,8 @@ static K1Variant<T> make_split_variant() {
 // developed field's cells close to its node box's anchor)
 template <typename T, int NT, int MINW, int FNT, int CPL, int FMINW = 1, int BZ = 16>
 static K1Variant<T> make_march_variant() {
-  K1Variant<T> v{0,   BZ,  NT, 0, nullptr, &k_potentials<T, NT, MINW, false>, nullptr, CPL, FNT,
-                 &k_fluxmarch<T, FNT, CPL, FMINW>, FMINW};
+  K1Variant<T> v{0, BZ, NT, &k_potentials<T, NT, MINW, false>, nullptr, CPL, FNT, &k_fluxmarch<T, FNT, CPL, FMINW>,
+                 FMINW};
   v.fpot_wide = &k_potentials<T, NT, MINW, true>;
   return v;
 }
@@ -920,30 +913,26 @@ static K1Variant<T> make_march_variant() {
 // rows and planes
 template <typename T, int NT, int MINW, int FNT, int FMINW>
 static K1Variant<T> make_fused_variant() {
-  K1Variant<T> v{0,   16,  NT, 0, nullptr, &k_potentials<T, NT, MINW, false>, nullptr, 2, FNT,
-                 &k_stencilmarch<T, FNT, FMINW>, FMINW};
+  K1Variant<T> v{0, 16, NT, &k_potentials<T, NT, MINW, false>, nullptr, 2, FNT, &k_stencilmarch<T, FNT, FMINW>,
+                 FMINW};
   v.fpot_wide = &k_potentials<T, NT, MINW, true>;
   v.fused = true;
   return v;
 }
 
-// stencil tilings the autotuner chooses from (BY x BZ tile, NT threads, min waves/SIMD)
+// stencil tilings the autotuner chooses from
 template <typename T>
 static std::vector<K1Variant<T>> k1_variants();
 template <>
 std::vector<K1Variant<double>> k1_variants<double>() {
-  return {make_variant<double, 8, 32, 256, 2>(), make_variant<double, 4, 32, 128, 2>(),
-          make_variant<double, 8, 16, 128, 3>(), make_split_variant<double, 256, 2, 0>(),
+  return {make_split_variant<double, 256, 2, 0>(),
           make_split_variant<double, 256, 2, 1>(), make_split_variant<double, 256, 2, 1, 32>(),
           make_split_variant<double, 256, 2, 2>(), make_march_variant<double, 256, 2, 256, 1>(),
           make_march_variant<double, 256, 2, 256, 1, 3>()};
 }
 template <>
 std::vector<K1Variant<float>> k1_variants<float>() {
-  return {make_variant<float, 8, 32, 256, 3>(), make_variant<float, 8, 32, 256, 2>(),
-          make_variant<float, 16, 32, 256, 2>(), make_variant<float, 8, 64, 256, 2>(),
-          make_variant<float, 4, 64, 256, 3>(), make_variant<float, 8, 16, 128, 4>(),
-          make_split_variant<float, 256, 3, 0>(), make_split_variant<float, 256, 4, 0>(),
+  return {make_split_variant<float, 256, 3, 0>(), make_split_variant<float, 256, 4, 0>(),
           make_split_variant<float, 256, 4, 1>(), make_split_variant<float, 256, 4, 1, 32>(),
           make_split_variant<float, 256, 4, 2>(), make_march_variant<float, 256, 4, 256, 1>(),
           make_march_variant<float, 256, 4, 256, 2>(), make_march_variant<float, 256, 4, 512, 2>(),
@@ -1051,14 +1040,10 @@ struct Engine : EngineBase {
     }
     d_done = (unsigned*)dmalloc(sizeof(unsigned));
     HIPCHK(hipMemset(d_done, 0, sizeof(unsigned)));
-    for (auto& v : variants)
-      if (!v.split())
-        (void)hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds);
     // default tiling (until set_tiling): the marching two-pass form with 16 x 16
     // potentials tiles, the flux pass with two cells per lane at two waves per
     // SIMD in FP32 and one cell at three in FP64, the autotuner's pick for 3D
-    // fields from 64^3 to 256^3 (about twice as fast there as the single-pass
-    // tilings); x chunks of 16 planes; FFT lines of about six points per thread
+    // fields from 64^3 to 256^3; x chunks of 16 planes; FFT lines of about six points per thread
     // (32 KiB, 256 threads) and two (FP32) or four (FP64) parts per plane of the
     // 128 x 128 plane passes
     const int want_cpl = sizeof(T) == 4 ? 2 : 1, want_minw = sizeof(T) == 4 ? 1 : 3;
@@ -1067,9 +1052,7 @@ struct Engine : EngineBase {
       if (variants[i].fcpl == want_cpl && variants[i].fminw == want_minw && variants[i].fnt == 256 &&
           variants[i].bz == 16)
         variant = i;
-    for (int i = 0; i < (int)variants.size() && variant < 0; ++i)
-      if (!variants[i].split() && variants[i].lds <= (size_t)devinfo().lds) variant = i;
-    require(variant >= 0, "no stencil tiling fits this device's LDS");
+    require(variant >= 0, "no default stencil tiling");
     default_variant = variant;
     fft.budget = 32 * 1024;
     fft.nt = 256;
@@ -1189,6 +1172,10 @@ struct Engine : EngineBase {
       require(d->shape[a] >= 4, "every axis needs at least 4 cells");
       ncell *= d->shape[a];
     }
+    // the stencil kernels address a simulation's (ncell, 4) fields with 32-bit
+    // byte offsets: up to 2^28 cells in FP32 (645^3), 2^27 in FP64 (512^3)
+    require(ncell * 4 * (long)sizeof(T) <= (1L << 32),
+            "field too large: at most " + std::to_string((1L << 32) / (4 * (long)sizeof(T))) + " cells per simulation");
     require(!d->elastic || bhat, "elastic simulation needs bhat");
     HIPCHK(hipStreamSynchronize(stream));
     int id = -1;
@@ -1325,7 +1312,7 @@ struct Engine : EngineBase {
     return v;
   }
   // a simulation's table misses rows of the simplex: its nearest-row search must
-  // skip them (the WIDE potentials pass; the single-pass stencil lacks it)
+  // skip them (the WIDE potentials pass)
   bool wide = false;
   int default_variant = 0;
   void (*pot(const K1Variant<T>& V) const)(const SimDev<T>*, const SimState*, const K1Item*, int) {
@@ -1400,8 +1387,6 @@ struct Engine : EngineBase {
     wide = false;
     for (const auto& s : sims)
       if (s.used && s.tab && !s.tab->complete) wide = true;
-    // the single-pass tilings search complete tables only: fall back to the default two-pass form
-    if (wide && !variants[variant].split()) variant = default_variant;
     // the fused marching form takes only the shapes it was built for
     if (variants[variant].fused && !fused_fits(variants[variant])) variant = default_variant;
     const K1Variant<T>& V = variants[variant];
@@ -1415,7 +1400,7 @@ struct Engine : EngineBase {
     // GROUP_BYTES of working set, at least one each
     std::vector<std::vector<int>> gsims;
     {
-      const bool can_group = V.split() && V.fcpl > 0;
+      const bool can_group = V.fcpl > 0;
       size_t acc = 0;
       for (int i = 0; i < (int)sims.size(); ++i) {
         if (!sims[i].used) continue;
@@ -1445,7 +1430,7 @@ struct Engine : EngineBase {
       const bool hx = v.dim == 3;
       if (V.fused) {
         fused_lists(i, v, V, k1, kf);
-      } else if (V.split()) {
+      } else {
         // tiles of at most nt cells: rows of up to bz cells (nt in 1D), as many rows as fit
         // slabs of planes (one per XCD, in the order the other kernels' lists
         // take them), then tile columns, x fastest: a block's contiguous range
@@ -1478,21 +1463,6 @@ struct Engine : EngineBase {
           }
         }
       }
-      const int by = (v.dim >= 2) ? V.by : 1;
-      const int cxs = hx ? std::max(1, std::min(xchunk, v.n0)) : 1;
-      for (int xa = 0; xa < (hx ? v.n0 : 1) && !V.split(); xa += cxs)
-        for (int y0 = 0; y0 < v.n1; y0 += by)
-          for (int z0 = 0; z0 < v.n2; z0 += V.bz) {
-            K1Item it;
-            it.sim = i;
-            it.xa = xa;
-            it.xb = hx ? std::min(xa + cxs, v.n0) : 1;
-            it.y0 = y0;
-            it.z0 = z0;
-            it.ty = std::min(by, v.n1 - y0);
-            it.tz = std::min(V.bz, v.n2 - z0);
-            k1.push_back(it);
-          }
       prange[2 * i] = (int)k2.size();
       for (long c0 = 0; c0 < s.ncell; c0 += K2_CHUNK) k2.push_back({i, c0, std::min(s.ncell, c0 + K2_CHUNK)});
       prange[2 * i + 1] = (int)k2.size();
@@ -1559,7 +1529,7 @@ struct Engine : EngineBase {
     }
     // K1 grid: persistent, at most what fits on the chip at once
     int occ = 1;
-    if (V.split()) {
+    {
       auto grid = [&](int o) { return xcd_grid(std::min(nk1, devinfo().cus * std::max(1, o))); };
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)pot(V), V.nt, 0));
       // (the fused form's potentials items are small and alike: one block each,
@@ -1577,10 +1547,6 @@ struct Engine : EngineBase {
         g->kf_grid = xcd_grid(std::max(1, std::min(g->nkf, devinfo().cus * std::max(1, occ))));
         g->k2_grid = xcd_grid(std::max(1, std::min(g->nk2, devinfo().cus * 8)));
       }
-    } else {
-      (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)V.fn, V.nt, V.lds);
-      occ = std::max(1, occ);
-      k1_grid = (unsigned)std::max(1, std::min(nk1, devinfo().cus * occ));
     }
     // prime after every rebuild (the partial-sum buffer is new): FP64 statistics
     // of the current X, and the misfit of fields just set; X and the other
@@ -1635,17 +1601,11 @@ struct Engine : EngineBase {
   bool launch_k1(bool with_dt = false) {
     if (!nk1) return false;
     const K1Variant<T>& V = variants[variant];
-    if (V.split()) {
-      hipLaunchKernelGGL(pot(V), dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
-      HIPCHK(hipGetLastError());
-      const bool did = launch_flux(V, with_dt);
-      HIPCHK(hipGetLastError());
-      return did;
-    }
-    require(V.lds <= (size_t)devinfo().lds, "K1 tile needs more LDS than a block may use");
-    hipLaunchKernelGGL(V.fn, dim3(k1_grid), dim3(V.nt), V.lds, stream, d_sims, d_state, d_xmax, d_k1, nk1);
+    hipLaunchKernelGGL(pot(V), dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
     HIPCHK(hipGetLastError());
-    return false;
+    const bool did = launch_flux(V, with_dt);
+    HIPCHK(hipGetLastError());
+    return did;
   }
   void enqueue_step() {
     fft.run(stream, d_state);
@@ -1862,7 +1822,6 @@ struct Engine : EngineBase {
     require(reps > 0 && which >= 0 && which <= 5, "time_kernel: bad arguments");
     if (dirty) rebuild();
     const K1Variant<T>& V = variants[variant];
-    require(which < 4 || V.split(), "time_kernel: the two-pass stencil is not selected");
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -1908,7 +1867,7 @@ struct Engine : EngineBase {
     if (dirty) rebuild();
     require(groups.size() <= 1, "time_step_stages: the simulations form more than one group");
     const K1Variant<T>& V = variants[variant];
-    require(V.split() && V.fcpl > 0, "time_step_stages: the marching two-pass stencil is not selected");
+    require(V.fcpl > 0, "time_step_stages: the marching two-pass stencil is not selected");
     require(nk1 > 0 && nk2 > 0, "time_step_stages: no active simulation");
     constexpr int NE = 6;
     std::vector<hipEvent_t> ev((size_t)NE * reps, nullptr);
@@ -1964,7 +1923,7 @@ struct Engine : EngineBase {
     out[1] = nk2;
     out[2] = fft.offs[5];
     out[3] = k1_grid;
-    out[4] = (long)variants[variant].lds;
+    out[4] = 0;  // (K1 LDS bytes: every stencil pass has static LDS only)
     out[5] = variant;
     out[6] = xchunk;
     out[7] = (long)fft.budget;

@@ -101,8 +101,8 @@ PF5D_DEV void sst(void* p, const V& v) {
 // Accesses at a 32-bit byte offset from a base the whole wavefront shares
 // (a simulation's field, in scalar registers): the compiler then issues
 // global_load/store with the base as saddr and the offset in one VGPR, with no
-// 64-bit address arithmetic per access.  Offsets stay below 4 GiB (fields of
-// fewer than 2^27 FP64 cells).
+// 64-bit address arithmetic per access.  Offsets stay below 4 GiB: the engine
+// admits simulations of at most 2^32 / (4 sizeof(T)) cells.
 template <typename V>
 PF5D_DEV V gldo(const void* base, unsigned off) {
   return gld<V>(static_cast<const char*>(base) + off);

@@ -1,7 +1,8 @@
-// Device-resident fused explicit step.
+// Device-resident fused explicit step: the types the step kernels share, and
+// the stages after the stencil (the stencil stage K1 -- interpolation,
+// kappa*lap(X) and elastic psi -> mu, Mob -> face fluxes -> dX/dt and the block
+// max |dX/dt| -- is in pf5d_split.h).
 //
-//   K1  k_stencil : interpolation + kappa*lap(X) + elastic psi -> mu, Mob (in LDS)
-//                   -> face fluxes (each face once) -> dX/dt, block max |dX/dt|
 //   DT  k_dtctl   : per simulation: max |dX/dt| -> adaptive dt (system.py:90-112),
 //                   time, step count, FP64 mass / bounds statistics, status
 //   K2  k_update  : X += dt dX/dt, simplex clamp, misfit s for the next FFT,
@@ -62,309 +63,6 @@ struct K2Item {
   int sim;
   long c0, c1;  // cell range
 };
-
-// ---------------------------------------------------------------------------
-// K1
-// ---------------------------------------------------------------------------
-template <typename T, int BY, int BZ>
-struct K1Smem {
-  static constexpr int XY = BY + 4, XZ = BZ + 4;  // X planes with halo 2
-  static constexpr int MY = BY + 2, MZ = BZ + 2;  // mu / Mob planes with halo 1
-  static constexpr size_t x_off = 0;
-  static constexpr size_t x_sz = sizeof(V4<T>) * 3 * XY * XZ;
-  static constexpr size_t mu_off = x_off + x_sz;
-  static constexpr size_t mu_sz = sizeof(V4<T>) * 2 * MY * MZ;
-  static constexpr size_t mb_off = mu_off + mu_sz;
-  static constexpr size_t mb_sz = sizeof(T) * 2 * 5 * MY * MZ;
-  static constexpr size_t fy_off = mb_off + mb_sz;
-  static constexpr size_t fy_sz = sizeof(V4<T>) * (BY + 1) * BZ;
-  static constexpr size_t fz_off = fy_off + fy_sz;
-  static constexpr size_t fz_sz = sizeof(V4<T>) * BY * (BZ + 1);
-  static constexpr size_t red_off = fz_off + fz_sz;
-  static constexpr size_t red_sz = sizeof(T) * 16;
-  // table nodes of the plane's interpolation box (rows of NPAD), staged once per plane
-  static constexpr int BOXMAX = sizeof(T) == 4 ? 256 : 128;
-  static constexpr size_t box_off = (red_off + red_sz + 15) / 16 * 16;
-  static constexpr size_t box_sz = sizeof(T) * TablePad<T>::value * BOXMAX;
-  static constexpr size_t bred_off = box_off + box_sz;      // int [16][8] wave partials + [8] result
-  static constexpr size_t bytes = bred_off + sizeof(int) * (16 * 8 + 8);
-};
-
-// MINW: minimum waves per SIMD the register allocation must allow
-template <typename T, int BY, int BZ, int NT, int MINW>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
-k_stencil(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T* __restrict__ xmax_acc,
-          const K1Item* __restrict__ items, int nitems) {
-  using S = K1Smem<T, BY, BZ>;
-  extern __shared__ __align__(16) unsigned char smem[];
-  V4<T>* XS = reinterpret_cast<V4<T>*>(smem + S::x_off);
-  V4<T>* MU = reinterpret_cast<V4<T>*>(smem + S::mu_off);
-  T* MB = reinterpret_cast<T*>(smem + S::mb_off);
-  V4<T>* FY = reinterpret_cast<V4<T>*>(smem + S::fy_off);
-  V4<T>* FZ = reinterpret_cast<V4<T>*>(smem + S::fz_off);
-  T* RED = reinterpret_cast<T*>(smem + S::red_off);
-  T* BOX = reinterpret_cast<T*>(smem + S::box_off);
-  int* BRED = reinterpret_cast<int*>(smem + S::bred_off);
-  constexpr int CPT = (BY * BZ + NT - 1) / NT;
-  constexpr int NP = TablePad<T>::value;
-  using VT = typename Vec16<T>::type;
-  constexpr int NCOL = NP / Vec16<T>::n;  // 16-byte columns per table row
-  const int tid = threadIdx.x;
-
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const K1Item item = items[it];
-    if (!(st[item.sim].status & ST_ACTIVE)) continue;  // uniform per block
-    const SimDev<T>& sd = sims[item.sim];
-    const bool hx = sd.dim == 3, hy = sd.dim >= 2;
-    const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
-    const int ty = item.ty, tz = item.tz;
-    const T inv_dx = sd.inv_dx, inv_dx2 = sd.inv_dx2;
-    const T cdim = T(2 * sd.dim);
-    // X region rows: j in [-2, ty+1] (hy) or [0,0]; cols k in [-2, tz+1]
-    const int xj0 = hy ? -2 : 0, xrows = hy ? ty + 4 : 1, xcols = tz + 4;
-    T tmax = T(0);
-    V4<T> fprev[CPT];
-
-    auto xslot = [&](int p) { return ((p % 3) + 3) % 3; };
-    auto XAT = [&](int slot, int j, int k) -> V4<T>& { return XS[(slot * S::XY + (j + 2)) * S::XZ + (k + 2)]; };
-    auto MUAT = [&](int slot, int j, int k) -> V4<T>& { return MU[(slot * S::MY + (j + 1)) * S::MZ + (k + 1)]; };
-    auto MBAT = [&](int slot, int m, int j, int k) -> T& {
-      return MB[((slot * 5 + m) * S::MY + (j + 1)) * S::MZ + (k + 1)];
-    };
-
-    // X planes go global -> registers (issued one iteration ahead) -> LDS
-    constexpr int XPT = ((BY + 4) * (BZ + 4) + NT - 1) / NT;
-    const float rx = frcp(xcols), rm = frcp(tz + 2), rt = frcp(tz),
-                rt1 = frcp(tz + 1);
-    V4<T> xr[XPT];
-    auto fetch_plane = [&](int p) {
-      const int gx = hx ? wrap(p, n0) : 0;
-      const int tot = xrows * xcols;
-#pragma unroll
-      for (int i = 0; i < XPT; ++i) {
-        const int idx = tid + i * NT;
-        if (idx < tot) {
-          int j, k;
-          fdivmod(idx, xcols, rx, j, k);
-          j += xj0;
-          k -= 2;
-          const int gy = hy ? wrap(item.y0 + j, n1) : 0;
-          const int gz = wrap(item.z0 + k, n2);
-          xr[i] = ld4g<T>(sd.X + 4 * (((long)gx * n1 + gy) * n2 + gz));
-        }
-      }
-    };
-    auto commit_plane = [&](int p) {
-      const int slot = xslot(p);
-      const int tot = xrows * xcols;
-#pragma unroll
-      for (int i = 0; i < XPT; ++i) {
-        const int idx = tid + i * NT;
-        if (idx < tot) {
-          int j, k;
-          fdivmod(idx, xcols, rx, j, k);
-          XAT(slot, j + xj0, k - 2) = xr[i];
-        }
-      }
-    };
-    auto load_plane = [&](int p) {
-      fetch_plane(p);
-      commit_plane(p);
-    };
-
-    // Table nodes used by the plane's linear-region points: the bounding box of
-    // their lower cube corners is reduced across the block (wave shuffles + LDS);
-    // when it holds at most BOXMAX nodes they are copied to LDS once and every
-    // point of the plane interpolates from there (same arithmetic, bit-identical),
-    // otherwise the points gather from the table in HBM.
-    auto stage_box = [&](int s0, int mj0, int mcols, int tot) -> NodeBox {
-      int v[8];
-#pragma unroll
-      for (int a = 0; a < 8; ++a) v[a] = 0x7fffffff;
-      for (int idx = tid; idx < tot; idx += NT) {
-        int j, k;
-        fdivmod(idx, mcols, rm, j, k);
-        j += mj0;
-        k -= 1;
-        if ((j < 0 || j >= ty) && (k < 0 || k >= tz)) continue;
-        const V4<T> xc = XAT(s0, j, k);
-        if (!(((xc.v[0] + xc.v[1]) + xc.v[2]) + xc.v[3] <= sd.tab.thresh)) continue;
-        node_box_point<T>(sd.tab, xc.v, v);
-      }
-      return stage_node_box<T, NT, S::BOXMAX>(sd.tab, v, BOX, BRED, tid);
-    };
-
-    // mu, Mob of plane p on the halo-1 region (corners excluded)
-    auto potentials = [&](int p) {
-      const int s0 = xslot(p), sp = xslot(p + 1), sm = xslot(p - 1), ms = ((p % 2) + 2) % 2;
-      const int gx = hx ? wrap(p, n0) : 0;
-      const int mj0 = hy ? -1 : 0, mrows = hy ? ty + 2 : 1, mcols = tz + 2;
-      const int tot = mrows * mcols;
-      const NodeBox nb = stage_box(s0, mj0, mcols, tot);
-      for (int idx = tid; idx < tot; idx += NT) {
-        int j, k;
-        fdivmod(idx, mcols, rm, j, k);
-        j += mj0;
-        k -= 1;
-        const bool jh = (j < 0 || j >= ty), kh = (k < 0 || k >= tz);
-        if (jh && kh) continue;
-        const V4<T> xc = XAT(s0, j, k);
-        T xa[4], P[9];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) xa[a] = xc.v[a];
-        interp9_staged<T>(sd.tab, BOX, nb, xa, P);
-        T acc[4];
-        bool first = true;
-        if (hx) {
-          const V4<T> u = XAT(sp, j, k), d = XAT(sm, j, k);
-#pragma unroll
-          for (int a = 0; a < 4; ++a) acc[a] = u.v[a] + d.v[a];
-          first = false;
-        }
-        if (hy) {
-          const V4<T> u = XAT(s0, j + 1, k), d = XAT(s0, j - 1, k);
-#pragma unroll
-          for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
-          first = false;
-        }
-        {
-          const V4<T> u = XAT(s0, j, k + 1), d = XAT(s0, j, k - 1);
-#pragma unroll
-          for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
-        }
-        T psi = T(0);
-        if (sd.elastic) {
-          const int gy = hy ? wrap(item.y0 + j, n1) : 0;
-          const int gz = wrap(item.z0 + k, n2);
-          psi = gld<T>(sd.psi + (((long)gx * n1 + gy) * n2 + gz));
-        }
-        V4<T> mu;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          // (acc - 2 dim x) / h^2, then P - L kappa: for FP64 each product and sum
-          // rounds separately (negation is exact), as in simulate.py:253-255
-          const T L = mad<T>(-cdim, xa[a], acc[a]) * inv_dx2;
-          T m = mad<T>(-L, sd.kappa[a], P[a]);
-          if (sd.elastic) m = mad<T>(psi, sd.l[a], m);
-          mu.v[a] = m;
-        }
-        MUAT(ms, j, k) = mu;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) MBAT(ms, m, j, k) = P[4 + m];
-      }
-    };
-
-    // flux through the face between (sa: j,k) and (sb: jb,kb)
-    auto face = [&](int xa_s, int ma_s, int ja, int ka, int xb_s, int mb_s, int jb, int kb, T* out) {
-      const V4<T> Xa = XAT(xa_s, ja, ka), Xb = XAT(xb_s, jb, kb);
-      const V4<T> ua = MUAT(ma_s, ja, ka), ub = MUAT(mb_s, jb, kb);
-      T Ma[5], Mb[5];
-#pragma unroll
-      for (int m = 0; m < 5; ++m) {
-        Ma[m] = MBAT(ma_s, m, ja, ka);
-        Mb[m] = MBAT(mb_s, m, jb, kb);
-      }
-      step_face_flux<T>(Xa.v, Xb.v, Ma, Mb, ua.v, ub.v, inv_dx, out);
-    };
-
-    const int pbeg = hx ? item.xa - 1 : 0, pend = hx ? item.xb : 0;
-    if (hx) {
-      load_plane(pbeg - 1);
-      load_plane(pbeg);
-      fetch_plane(pbeg + 1);
-    } else {
-      load_plane(0);
-    }
-    for (int p = pbeg; p <= pend; ++p) {
-      if (hx) {
-        commit_plane(p + 1);                // fetched during the previous iteration
-        if (p + 2 <= pend + 1) fetch_plane(p + 2);  // in flight while this plane computes
-      }
-      __syncthreads();
-      potentials(p);
-      __syncthreads();
-      const int q = hx ? p - 1 : p;  // output plane of this iteration
-      const bool out_ok = hx ? (p - 1 >= item.xa) : true;
-      V4<T> fx[CPT];
-      if (hx && p >= item.xa) {
-        // x face (p-1 -> p) for interior cells
-#pragma unroll
-        for (int i = 0; i < CPT; ++i) {
-          const int c = tid + i * NT;
-          if (c < ty * tz) {
-            int j, k;
-            fdivmod(c, tz, rt, j, k);
-            face(xslot(p - 1), ((p - 1) % 2 + 2) % 2, j, k, xslot(p), (p % 2 + 2) % 2, j, k, fx[i].v);
-          }
-        }
-      }
-      if (out_ok) {
-        const int xs = xslot(q), ms = ((q % 2) + 2) % 2;
-        if (hy) {
-          for (int idx = tid; idx < (ty + 1) * tz; idx += NT) {
-            int j, k;
-            fdivmod(idx, tz, rt, j, k);
-            face(xs, ms, j - 1, k, xs, ms, j, k, FY[j * BZ + k].v);
-          }
-        }
-        for (int idx = tid; idx < ty * (tz + 1); idx += NT) {
-          int j, k;
-          fdivmod(idx, tz + 1, rt1, j, k);
-          face(xs, ms, j, k - 1, xs, ms, j, k, FZ[j * (BZ + 1) + k].v);
-        }
-        __syncthreads();
-        const int gx = hx ? wrap(q, n0) : 0;
-#pragma unroll
-        for (int i = 0; i < CPT; ++i) {
-          const int c = tid + i * NT;
-          if (c < ty * tz) {
-            int j, k;
-            fdivmod(c, tz, rt, j, k);
-            const V4<T> zp = FZ[j * (BZ + 1) + k + 1], zm = FZ[j * (BZ + 1) + k];
-            T acc[4];
-            bool first = true;
-            if (hx) {
-#pragma unroll
-              for (int a = 0; a < 4; ++a) acc[a] = fx[i].v[a] - fprev[i].v[a];
-              first = false;
-            }
-            if (hy) {
-              const V4<T> yp = FY[(j + 1) * BZ + k], ym = FY[j * BZ + k];
-#pragma unroll
-              for (int a = 0; a < 4; ++a) acc[a] = first ? (yp.v[a] - ym.v[a]) : ((acc[a] + yp.v[a]) - ym.v[a]);
-              first = false;
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a) acc[a] = first ? (zp.v[a] - zm.v[a]) : ((acc[a] + zp.v[a]) - zm.v[a]);
-            V4<T> o;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-              o.v[a] = acc[a] * step_div_scale<T>(inv_dx);
-              tmax = max_ignore_nan<T>(tmax, fabs(o.v[a]));
-            }
-            const int gy = hy ? item.y0 + j : 0;
-            st4g<T>(sd.dX + 4 * (((long)gx * n1 + gy) * n2 + item.z0 + k), o);
-          }
-        }
-      }
-      if (hx) {
-#pragma unroll
-        for (int i = 0; i < CPT; ++i) fprev[i] = fx[i];
-      }
-      __syncthreads();
-    }
-    // block max -> one atomic per item
-    T m = wave_max<T>(tmax);
-    if ((tid & 63) == 0) RED[tid >> 6] = m;
-    __syncthreads();
-    if (tid == 0) {
-      T r = RED[0];
-      for (int w = 1; w < NT / 64; ++w) r = (RED[w] > r) ? RED[w] : r;
-      atomic_max_nonneg(&xmax_acc[item.sim], r);
-    }
-    __syncthreads();
-  }
-}
 
 // ---------------------------------------------------------------------------
 // DT: one thread block per simulation

@@ -45,10 +45,6 @@ namespace pf5d {
   PF5D_FRP_INSTANCE(double, 4, 2) \
   PF5D_FRX_INSTANCE(float, 1)     \
   PF5D_FRX_INSTANCE(double, 1)
-#define PF5D_STENCIL_INSTANCE(T, BY, BZ, NT, MINW)                                                 \
-  PF5D_INST __global__ void k_stencil<T, BY, BZ, NT, MINW>(const SimDev<T>* __restrict__,          \
-                                                           SimState* __restrict__, T* __restrict__, \
-                                                           const K1Item* __restrict__, int);
 #define PF5D_POTENTIALS_INSTANCE(T, NT, MINW)                                                                   \
   PF5D_INST __global__ void k_potentials<T, NT, MINW, false>(const SimDev<T>* __restrict__,                       \
                                                              const SimState* __restrict__, const K1Item* __restrict__, \
@@ -106,17 +102,8 @@ namespace pf5d {
   PF5D_FFTP_INSTANCES_T(float) \
   PF5D_FFTP_INSTANCES_T(double)
 
-// the stencil tilings and two-pass forms the engine offers the autotuner
+// the stencil forms the engine offers the autotuner
 #define PF5D_STENCIL_INSTANCES                    \
-  PF5D_STENCIL_INSTANCE(double, 8, 32, 256, 2)    \
-  PF5D_STENCIL_INSTANCE(double, 4, 32, 128, 2)    \
-  PF5D_STENCIL_INSTANCE(double, 8, 16, 128, 3)    \
-  PF5D_STENCIL_INSTANCE(float, 8, 32, 256, 3)     \
-  PF5D_STENCIL_INSTANCE(float, 8, 32, 256, 2)     \
-  PF5D_STENCIL_INSTANCE(float, 16, 32, 256, 2)    \
-  PF5D_STENCIL_INSTANCE(float, 8, 64, 256, 2)     \
-  PF5D_STENCIL_INSTANCE(float, 4, 64, 256, 3)     \
-  PF5D_STENCIL_INSTANCE(float, 8, 16, 128, 4)     \
   PF5D_POTENTIALS_INSTANCE(double, 256, 2)        \
   PF5D_POTENTIALS_INSTANCE(float, 256, 3)         \
   PF5D_POTENTIALS_INSTANCE(float, 256, 4)         \

@@ -171,28 +171,39 @@ PF5D_DEV void interp_core(const T* base, int n1, int n2, int n3, T t0, T t1, T t
   for (int r = 0; r < 8; ++r) rp[r] = base + ((r >> 2) & 1) * n3 + ((r >> 1) & 1) * n2 + (r & 1) * n1;
   T w[16];
   corner_weights<T>(t0, t1, t2, t3, w);
-  // full 16-byte batches, then the ninth property alone (no padding fetched)
+  // full 16-byte batches, then the ninth property alone (no padding fetched).
+  // From LDS a batch reads all 16 corners at once; from HBM (a path beside the
+  // LDS one in the same kernels, taken rarely) it reads them in two halves, the
+  // sums running on across them in the same order, so that it needs no more
+  // registers than the LDS path
+  constexpr int HB = FROM_LDS ? 16 : 8;
 #pragma unroll
   for (int g = 0; g < 8 / VN; ++g) {
-    VT c[16];
+    T acc[VN];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      const T* q = rp[b >> 1] + (b & 1) * NP + g * VN;
-      if constexpr (FROM_LDS)
-        c[b] = sld<VT>(q);
-      else
-        c[b] = gld<VT>(q);
+    for (int h = 0; h < 16 / HB; ++h) {
+      VT c[HB];
+#pragma unroll
+      for (int b = 0; b < HB; ++b) {
+        const int k = HB * h + b;
+        const T* q = rp[k >> 1] + (k & 1) * NP + g * VN;
+        if constexpr (FROM_LDS)
+          c[b] = sld<VT>(q);
+        else
+          c[b] = gld<VT>(q);
+      }
+#pragma unroll
+      for (int e = 0; e < VN; ++e) {
+        if (h == 0) acc[e] = w[0] * vget(c[0], e);
+#pragma unroll
+        for (int b = h == 0 ? 1 : 0; b < HB; ++b) acc[e] = mad<T>(w[HB * h + b], vget(c[b], e), acc[e]);
+      }
+      // keep the scheduler from hoisting the next loads above these sums: at
+      // most one batch of rows is held in registers at a time
+      __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int e = 0; e < VN; ++e) {
-      T acc = w[0] * vget(c[0], e);
-#pragma unroll
-      for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], vget(c[b], e), acc);
-      P[g * VN + e] = acc;
-    }
-    // keep the scheduler from hoisting the next batch's loads above this one's
-    // sums: at most 16 rows are held in registers at a time
-    __builtin_amdgcn_sched_barrier(0);
+    for (int e = 0; e < VN; ++e) P[g * VN + e] = acc[e];
   }
   {
     T c[16];
@@ -212,7 +223,10 @@ PF5D_DEV void interp_core(const T* base, int n1, int n2, int n3, T t0, T t1, T t
   (void)NB;
 }
 
-// interp_core's arithmetic on one cube of the cube table (contiguous corners)
+// interp_core's arithmetic on one cube of the cube table (contiguous corners).
+// Each 16-byte column is read in two halves of eight corners (the sums run on
+// across them in the same order): half the registers of interp_core's batches,
+// so the kernels that carry this path beside their LDS path do not spill
 template <typename T>
 PF5D_DEV void interp_cube(const T* cube, T t0, T t1, T t2, T t3, T P[9]) {
   using VT = typename Vec16<T>::type;
@@ -221,17 +235,22 @@ PF5D_DEV void interp_cube(const T* cube, T t0, T t1, T t2, T t3, T P[9]) {
   corner_weights<T>(t0, t1, t2, t3, w);
 #pragma unroll
   for (int g = 0; g < 8 / VN; ++g) {
-    VT c[16];
+    T acc[VN];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) c[b] = gld<VT>(cube + (g * 16 + b) * VN);
+    for (int h = 0; h < 2; ++h) {
+      VT c[8];
 #pragma unroll
-    for (int e = 0; e < VN; ++e) {
-      T acc = w[0] * vget(c[0], e);
+      for (int b = 0; b < 8; ++b) c[b] = gld<VT>(cube + (g * 16 + 8 * h + b) * VN);
 #pragma unroll
-      for (int b = 1; b < 16; ++b) acc = mad<T>(w[b], vget(c[b], e), acc);
-      P[g * VN + e] = acc;
+      for (int e = 0; e < VN; ++e) {
+        if (h == 0) acc[e] = w[0] * vget(c[0], e);
+#pragma unroll
+        for (int b = h == 0 ? 1 : 0; b < 8; ++b) acc[e] = mad<T>(w[8 * h + b], vget(c[b], e), acc[e]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
-    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int e = 0; e < VN; ++e) P[g * VN + e] = acc[e];
   }
   T c[16];
 #pragma unroll
@@ -269,55 +288,6 @@ PF5D_DEV void interp_linear(const TableView<T>& tv, const T X[4], T P[9]) {
   axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
   interp_gather<T>(tv, i0, i1, i2, i3, t0, t1, t2, t3, P);
   (void)NP;
-}
-
-// A box of table nodes [lo_d, lo_d + ext_d) staged in LDS, rows of NPAD elements
-// in the table's own row-major node order.
-struct NodeBox {
-  int lo[4];
-  int st[4];   // node strides: st[3] = 1, st[2] = ext3, st[1] = ext3*ext2, st[0] = ext3*ext2*ext1
-  int vol;     // number of nodes (0: not staged)
-};
-
-template <typename T>
-PF5D_DEV void interp_linear_box(const TableView<T>& tv, const T* box, const NodeBox& nb, const T X[4], T P[9]) {
-  constexpr int NP = TablePad<T>::value;
-  const int num = tv.num;
-  int i0, i1, i2, i3;
-  T t0, t1, t2, t3;
-  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
-  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
-  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
-  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
-  const int q = (i0 - nb.lo[0]) * nb.st[0] + (i1 - nb.lo[1]) * nb.st[1] + (i2 - nb.lo[2]) * nb.st[2] + (i3 - nb.lo[3]);
-  interp_core<T, true>(box + q * NP, nb.st[2] * NP, nb.st[1] * NP, nb.st[0] * NP, t0, t1, t2, t3, P);
-}
-
-// interp9 with the linear region read from an LDS node box when one is staged
-// (nb.vol > 0), else gathered from the table in HBM; the same arithmetic either
-// way, with LDS (ds_read) or global loads chosen at compile time
-template <typename T>
-PF5D_DEV void interp9_staged(const TableView<T>& tv, const T* box, const NodeBox& nb, const T X[4], T P[9]) {
-  constexpr int NP = TablePad<T>::value;
-  const T s = ((X[0] + X[1]) + X[2]) + X[3];
-  if (!(s <= tv.thresh)) {
-    interp_nearest<T, false>(tv, X, P);
-    return;
-  }
-  const int num = tv.num;
-  int i0, i1, i2, i3;
-  T t0, t1, t2, t3;
-  axis_index<T>(X[0] * tv.inv_step, num, i0, t0);
-  axis_index<T>(X[1] * tv.inv_step, num, i1, t1);
-  axis_index<T>(X[2] * tv.inv_step, num, i2, t2);
-  axis_index<T>(X[3] * tv.inv_step, num, i3, t3);
-  if (nb.vol > 0) {
-    const T* base = box + ((i0 - nb.lo[0]) * nb.st[0] + (i1 - nb.lo[1]) * nb.st[1] + (i2 - nb.lo[2]) * nb.st[2] +
-                           (i3 - nb.lo[3])) * NP;
-    interp_core<T, true>(base, nb.st[2] * NP, nb.st[1] * NP, nb.st[0] * NP, t0, t1, t2, t3, P);
-  } else {
-    interp_gather<T>(tv, i0, i1, i2, i3, t0, t1, t2, t3, P);
-  }
 }
 
 // A fixed box of 4^4 table nodes with lower node lo[], staged in LDS: points
@@ -381,81 +351,6 @@ PF5D_DEV void interp9_box4(const TableView<T>& tv, const T* box, const int lo[4]
   }
 }
 
-// floor index of the lower cube corner on one axis (same rule as axis_index)
-template <typename T>
-PF5D_DEV int floor_index(T u, int num) {
-  return (u <= T(0)) ? 0 : ((u >= T(num - 1)) ? num - 2 : (int)u);
-}
-
-// fold one linear-region point into a thread's box bounds: v[a] = min lower
-// corner index on axis a, v[4 + a] = min of its negation (start from INT_MAX)
-template <typename T>
-PF5D_DEV void node_box_point(const TableView<T>& tab, const T X[4], int v[8]) {
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int f = floor_index<T>(X[a] * tab.inv_step, tab.num);
-    v[a] = min(v[a], f);
-    v[4 + a] = min(v[4 + a], -f);
-  }
-}
-
-// Block-wide node box: the bounds in v are reduced over the block (wave
-// shuffles, then one partial per wave in bred, an int[(NT/64)*8 + 8] LDS
-// array); the box spans [min, max + 1] on every axis.  With at most BOXMAX
-// nodes its rows are copied from the table to box (LDS, rows of NPAD) and the
-// box is returned, otherwise vol = 0 (points then gather from the table).
-// Every thread of the block calls it: it synchronises the block.
-template <typename T, int NT, int BOXMAX>
-PF5D_DEV NodeBox stage_node_box(const TableView<T>& tab, int v[8], T* box, int* bred, int tid) {
-  constexpr int NP = TablePad<T>::value;
-  using VT = typename Vec16<T>::type;
-  constexpr int NCOL = NP / Vec16<T>::n;  // 16-byte columns per row
-  constexpr int NW = NT / 64;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int a = 0; a < 8; ++a) v[a] = min(v[a], __shfl_xor(v[a], off, 64));
-  if ((tid & 63) == 0)
-#pragma unroll
-    for (int a = 0; a < 8; ++a) bred[(tid >> 6) * 8 + a] = v[a];
-  __syncthreads();
-  if (tid < 8) {
-    int m = bred[tid];
-    for (int w = 1; w < NW; ++w) m = min(m, bred[w * 8 + tid]);
-    bred[NW * 8 + tid] = m;
-  }
-  __syncthreads();
-  NodeBox nb;
-  nb.vol = 1;
-  int ext[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int lo = bred[NW * 8 + a], hi = -bred[NW * 8 + 4 + a];
-    nb.lo[a] = lo;
-    ext[a] = (lo > hi) ? 0 : hi - lo + 2;
-    nb.vol *= ext[a];
-  }
-  if (nb.vol == 0 || nb.vol > BOXMAX) {
-    nb.vol = 0;
-    return nb;
-  }
-  nb.st[3] = 1;
-  nb.st[2] = ext[3];
-  nb.st[1] = ext[3] * ext[2];
-  nb.st[0] = nb.st[1] * ext[1];
-  const int num = tab.num;
-#pragma unroll 1
-  for (int e = tid; e < nb.vol * NCOL; e += NT) {
-    const int node = e / NCOL, col = e - node * NCOL;
-    const int q0 = node / nb.st[0], r0 = node - q0 * nb.st[0];
-    const int q1 = r0 / nb.st[1], r1 = r0 - q1 * nb.st[1];
-    const int q2 = r1 / nb.st[2], q3 = r1 - q2 * nb.st[2];
-    const long row = ((((long)(nb.lo[0] + q0) * num + nb.lo[1] + q1) * num + nb.lo[2] + q2) * num + nb.lo[3] + q3);
-    reinterpret_cast<VT*>(box)[e] = gld<VT>(tab.data + row * NP + col * Vec16<T>::n);
-  }
-  __syncthreads();
-  return nb;
-}
 
 // Nearest table row (the reference's KD-tree query on the table rows,
 // interpolation.py:203-209).  Rows are the nodes of the dense grid that hold
@@ -542,7 +437,9 @@ PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
     near_sum += r;
   }
   near_in = near_sum <= num - 1;
-  double best = 1e300;
+  // (infinity, not a large finite bound: its low word is zero, so it is an
+  // instruction literal instead of a register pair held across the caller's loop)
+  double best = __builtin_huge_val();
   long bidx = -1;
   if (finite) {
     nearest_scan<T, WIDE>(tv, X, lo, 1, 3, best, bidx);
@@ -551,13 +448,13 @@ PF5D_DEV void interp_nearest(const TableView<T>& tv, const T X[4], T P[9]) {
 #pragma unroll 1
       while (!(bidx >= 0 && best <= (double)((R + 1) * (R + 1)) * (h * h)) && R <= 2 * num) {
         R = (R == 0) ? 1 : 2 * R;
-        best = 1e300;
+        best = __builtin_huge_val();
         bidx = -1;
         nearest_scan<T, WIDE>(tv, X, lo, 1 - R, 3 + R, best, bidx);
       }
     } else {
       if (!near_in) {
-        best = 1e300;
+        best = __builtin_huge_val();
         bidx = -1;
         nearest_scan<T, WIDE>(tv, X, lo, 0, 4, best, bidx);
       }

@@ -1,28 +1,29 @@
-// Two-pass form of the stencil stage: the same arithmetic as k_stencil,
-// operation for operation, so both forms give the same bits.
+// The stencil stage K1: interpolation + kappa lap(X) + elastic psi -> mu, Mob
+// -> face fluxes -> dX/dt and max |dX/dt| per simulation, in one of two forms
+// that give the same bits.
 //
+// Two-pass form:
 //   KP  k_potentials : per cell: interpolation + kappa lap(X) + elastic psi
 //                      -> mu (4 values) and Mob (5 values) in HBM; the table
-//                      nodes around the tile's first cell (4^4) are staged in
+//                      nodes around the tile's centre cell (4^4) are staged in
 //                      LDS and serve every point whose cube lies among them
-//   KF  k_fluxdiv    : per cell: the face fluxes to its neighbours from
-//                      (X, mu, Mob) -> dX/dt, max |dX/dt| per simulation
+//   KF  k_fluxmarch  : marching along axis 0 over tiles of rows: the face
+//                      fluxes from (X, mu, Mob), each face once -> dX/dt
+//                      (k_fluxdiv: the same per plane, each face from both
+//                      cells, for the per-plane tilings)
+// Fused marching form (k_stencilmarch): the marching pass computes mu and Mob
+// of its tile's inner rows itself, one plane ahead, into LDS, and reads them
+// from HBM only for the rows and planes on its tiles' boundaries, which KP
+// computes.
 //
-// The single-pass form keeps mu and Mob in LDS but recomputes them on a halo
-// and synchronises the block several times per plane; this form moves 9
-// values per cell through HBM twice and needs one block barrier per tile (the
-// node box), so many more waves stay resident.  Which is faster depends on
-// the shape mix; the autotuner times both.
-//
-// Work items are (sim, x planes xa..xb-1, y0, z0, ty, tz) tiles of at most NT
-// cells per plane: thread t handles cells (x, y0 + t / tz, z0 + t % tz) for x
-// in [xa, xb), all served by one staged node box (KP; the per-plane flux pass
-// takes single-plane items).  Items are listed with x fastest and every block
-// takes a contiguous range of them, so a block walks consecutive planes of one
-// tile and finds its x neighbours in cache.  A face flux is
-// evaluated by both cells it separates, always with the lower cell first, so
-// both see the same value (the divergence telescopes exactly, as in the
-// single-pass form where each face is evaluated once).
+// Work items of KP are (sim, x planes xa..xb-1, y0, z0, ty, tz) tiles of at
+// most NT cells per plane: thread t handles cells (x, y0 + t / tz, z0 + t % tz)
+// for x in [xa, xb), all served by one staged node box.  Items are listed with
+// x fastest and every block takes a contiguous range of them, so a block walks
+// consecutive planes of one tile and finds its x neighbours in cache.  A face
+// flux is evaluated with the lower cell first, so a face evaluated once (the
+// marching passes) or by both its cells (k_fluxdiv) gives the same value, and
+// the divergence telescopes exactly.
 #pragma once
 #include "pf5d_common.h"
 #include "pf5d_flux.h"
@@ -113,21 +114,31 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
   __shared__ __align__(16) T BOX[L::SLOTS * Vec16<T>::n];
   int ib, ie;
   block_range(nitems, ib, ie);
+  // the thread index is rebuilt per item from the wavefront index (a scalar)
+  // and the lane count below it (every lane is active at the loop's top), so no
+  // vector register holds it across the items
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   for (int it = ib; it < ie; ++it) {
     const K1Item item = items[it];
     if (!(st[item.sim].status & ST_ACTIVE)) continue;  // uniform per block
     // an empty asm keeps address arithmetic from being hoisted out of the item
     // loop (holding it across items costs more registers than recomputing it)
-    int tid = threadIdx.x;
+    unsigned ones = ~0u;
+    asm volatile("" : "+s"(ones));  // (opaque, so the lane count is not hoisted out of the loop)
+    int tid = wave * 64 + (int)__builtin_amdgcn_mbcnt_hi(ones, __builtin_amdgcn_mbcnt_lo(ones, 0u));
     asm volatile("" : "+v"(tid));
     const SimDev<T>& sd = sims[item.sim];
     const int n0 = sd.n0, n1 = sd.n1, n2 = sd.n2;
     const bool hx = sd.dim == 3, hy = sd.dim >= 2;
     int gx, gy, gz;
     const bool valid = tile_cell<T>(sd, item, tid, frcp(item.tz), gx, gy, gz);
-    long c = ((long)gx * n1 + gy) * n2 + gz;
+    // byte offsets of the cell in the (ncell, 4) and (ncell) fields, 32-bit
+    // (the engine admits at most 2^32 / (4 sizeof(T)) cells; the unsigned sums
+    // below wrap modulo 2^32 to the in-range offsets they stand for)
+    constexpr unsigned B4 = sizeof(V4<T>), B1 = sizeof(T);
+    unsigned c = ((unsigned)gx * n1 + gy) * n2 + gz;
     V4<T> xc{};
-    if (valid) xc = ld4g<T>(sd.X + 4 * c);
+    if (valid) xc = gldo<V4<T>>(sd.X, c * B4);
     int lo[4];
     stage_box4<T, NT>(sd, item, BOX, lo, tid);
     __syncthreads();
@@ -136,14 +147,20 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
     // over from the previous plane; FP64 reloads them (carrying them costs it
     // registers the fallback paths need)
     constexpr bool carry = sizeof(T) == 4;
-    const long sx = (long)n1 * n2;
+    const unsigned sx = (unsigned)n1 * n2;
+    // neighbour offsets in cells (periodic): x below and above, y, z
+    const unsigned dxd = gx > 0 ? 0u - sx : (unsigned)(n0 - 1) * sx;
+    const int dyu = (wrap(gy + 1, n1) - gy) * n2, dyd = (wrap(gy - 1, n1) - gy) * n2;
+    const int dzu = wrap(gz + 1, n2) - gz, dzd = wrap(gz - 1, n2) - gz;
     V4<T> xm{};
-    if (carry && valid && hx) xm = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -sx : (long)(n0 - 1) * sx)));
+    if (carry && valid && hx) xm = gldo<V4<T>>(sd.X, (c + dxd) * B4);
 #pragma unroll 1
     for (gx = item.xa; valid && gx < item.xb; ++gx, c += sx) {
-      if (!carry && gx > item.xa) xc = ld4g<T>(sd.X + 4 * c);
+      const unsigned c4 = c * B4;
+      const unsigned dxu = gx + 1 < n0 ? sx : sx - (unsigned)n0 * sx;
+      if (!carry && gx > item.xa) xc = gldo<V4<T>>(sd.X, c4);
       V4<T> xp{};
-      if (carry && hx) xp = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx)));
+      if (carry && hx) xp = gldo<V4<T>>(sd.X, (c + dxu) * B4);
       T P[9];
       interp9_box4<T, WIDE>(sd.tab, BOX, lo, xc.v, P);
       // neighbour sums in the reference order: x, then y, then z (each up + down)
@@ -154,34 +171,33 @@ k_potentials(const SimDev<T>* __restrict__ sims, const SimState* __restrict__ st
 #pragma unroll
           for (int a = 0; a < 4; ++a) acc[a] = xp.v[a] + xm.v[a];
         } else {
-          const V4<T> u = ld4g<T>(sd.X + 4 * (c + (gx + 1 < n0 ? sx : sx - (long)n0 * sx)));
-          const V4<T> d = ld4g<T>(sd.X + 4 * (c + (gx > 0 ? -sx : (long)(n0 - 1) * sx)));
+          const V4<T> u = gldo<V4<T>>(sd.X, (c + dxu) * B4);
+          const V4<T> d = gldo<V4<T>>(sd.X, (c + (gx > 0 ? 0u - sx : (unsigned)(n0 - 1) * sx)) * B4);
 #pragma unroll
           for (int a = 0; a < 4; ++a) acc[a] = u.v[a] + d.v[a];
         }
         first = false;
       }
       if (hy) {
-        const long u_i = c + (long)(wrap(gy + 1, n1) - gy) * n2, d_i = c + (long)(wrap(gy - 1, n1) - gy) * n2;
-        const V4<T> u = ld4g<T>(sd.X + 4 * u_i), d = ld4g<T>(sd.X + 4 * d_i);
+        const V4<T> u = gldo<V4<T>>(sd.X, c4 + dyu * (int)B4), d = gldo<V4<T>>(sd.X, c4 + dyd * (int)B4);
 #pragma unroll
         for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
         first = false;
       }
       {
-        const V4<T> u = ld4g<T>(sd.X + 4 * (c + (wrap(gz + 1, n2) - gz)));
-        const V4<T> d = ld4g<T>(sd.X + 4 * (c + (wrap(gz - 1, n2) - gz)));
+        const V4<T> u = gldo<V4<T>>(sd.X, c4 + dzu * (int)B4);
+        const V4<T> d = gldo<V4<T>>(sd.X, c4 + dzd * (int)B4);
 #pragma unroll
         for (int a = 0; a < 4; ++a) acc[a] = first ? (u.v[a] + d.v[a]) : ((acc[a] + u.v[a]) + d.v[a]);
       }
-      const T psi = sd.elastic ? gld<T>(sd.psi + c) : T(0);
+      const T psi = sd.elastic ? gldo<T>(sd.psi, c * B1) : T(0);
       const V4<T> mu = cell_mu<T>(sd, xc, acc, P, psi);
       V4<T> mb;
 #pragma unroll
       for (int a = 0; a < 4; ++a) mb.v[a] = P[4 + a];
-      gst_field<T, V4<T>>(sd.mu + 4 * c, mu);
-      gst_field<T, V4<T>>(sd.mb + 4 * c, mb);
-      gst_field<T, T>(sd.mb1 + c, P[8]);
+      gst_field_o<T, V4<T>>(sd.mu, c4, mu);
+      gst_field_o<T, V4<T>>(sd.mb, c4, mb);
+      gst_field_o<T, T>(sd.mb1, c * B1, P[8]);
       if (carry) {
         xm = xc;
         xc = xp;
@@ -533,7 +549,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
       kon[j] = row_on && k < tz;
       kz[j] = item.z0 + min(k, tz - 1);
     }
-    // cell indices in 32 bits (the host keeps fields below 2^27 cells)
+    // cell indices in 32 bits (the engine admits at most 2^32 / (4 sizeof(T)) cells)
     auto row_base = [&](int x, int y) { return ((unsigned)x * n1 + y) * n2; };
     auto load_lane = [&](int x, int y, CellV<T>(&c)[CPL]) {
       const unsigned r = row_base(x, y);
@@ -861,7 +877,7 @@ k_stencilmarch(const SimDev<T>* __restrict__ sims, SimState* __restrict__ st, T*
     const int gy = item.y0 + w;
     const bool inner = w > 0 && w < NW - 1;  // rows whose potentials are computed here
     auto xw = [&](int x) { return x < 0 ? x + n0 : (x >= n0 ? x - n0 : x); };
-    // cell indices in 32 bits (the host keeps fields below 2^27 cells)
+    // cell indices in 32 bits (the engine admits at most 2^32 / (4 sizeof(T)) cells)
     auto row_base = [&](int x, int y) { return ((unsigned)x * n1 + y) * n2 + CPL * lane; };
     auto load_full = [&](int x, int y, CellV<T>(&c)[CPL]) {
       const unsigned r = row_base(x, y);

@@ -31,7 +31,7 @@ STAGES = (
     ("k_frx", "fft_x"), ("k_fft<", "fft_x"),
     ("k_frp_inv", "fft_inv"), ("k_fftp_inv", "fft_inv"),
     ("k_potentials", "potentials"), ("k_fluxmarch", "flux"), ("k_fluxdiv", "flux"), ("k_stencilmarch", "flux"),
-    ("k_update<", "update"), ("k_stencil", "stencil"), ("k_dtctl", "dt"),
+    ("k_update<", "update"), ("k_dtctl", "dt"),
 )
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 

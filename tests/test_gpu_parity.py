@@ -223,29 +223,24 @@ def test_elastic_update(dev, dim, shape):
 
 
 def _split_variant(eng):
-    """Index of the two-pass stencil form among the engine's tilings (by == 0)."""
-    return next(i for i, v in enumerate(eng.variants()) if v[0] == 0)
+    """Index of the per-plane two-pass stencil form (flux pass without marching)."""
+    return next(i for i, v in enumerate(eng.variants()) if v[0] == 0 and v[3] == 0)
 
 
-def _fused_variant(eng):
-    """Index of the first single-pass stencil tiling (by > 0)."""
-    return next(i for i, v in enumerate(eng.variants()) if v[0] > 0)
-
-
-def _engine_one_step(pb, precision="fp64", form="fused"):
+def _engine_one_step(pb, precision="fp64", form="march"):
+    """One step; form "march": the engine's default tiling (marching flux pass),
+    "split": the per-plane flux pass."""
     from phasefield5d_amd.engine import Engine
     eng = Engine(precision)
     slot = eng.add_problem(pb)
     if form == "split":
         eng.set_tiling(_split_variant(eng), 1)
-    else:
-        eng.set_tiling(_fused_variant(eng), 16)
     eng.step(1)
     eng.sync()
     return eng, slot
 
 
-@pytest.mark.parametrize("form", ["fused", "split"])
+@pytest.mark.parametrize("form", ["march", "split"])
 @pytest.mark.parametrize("dim,n,elastic", [(1, 160, True), (1, 3200, True), (2, 32, True), (2, 32, False),
                                            (3, 16, True), (3, 24, False)])
 def test_fused_step_fp64(dev, dim, n, elastic, form):
@@ -315,6 +310,36 @@ def test_fp32_engine_error(dev):
     drift = np.abs(st["mass"] - m0) / m0
     print("fp32 relative mass drift after 50 steps:", drift)
     assert drift.max() < 1e-12
+
+
+def test_fp32_state_remainder_roundtrip(dev):
+    """The FP32 engine holds a field as the nearest float plus a 16-bit remainder
+    in units of 2^-15 of the float spacing: set_field / field round-trips every
+    value to within 2^-16 of that spacing (exactly where the value has at most
+    39 significant bits), across magnitudes, signs, zeros, subnormal floats and
+    values on either side of powers of two."""
+    from phasefield5d_amd.engine import Engine
+    pb = problem(2, 16, elastic=False)
+    rng = np.random.default_rng(11)
+    v = rng.uniform(-0.1, 1.1, size=16 * 16 * 4)
+    specials = [0.0, -0.0, 1.0, 0.5, 0.25, 2.0 ** -20, -2.0 ** -20, 1e-30, -1e-30, 1e-45, 3e-39,
+                np.nextafter(0.5, 0), np.nextafter(0.5, 1), np.nextafter(0.25, 0), 1.0 - 2.0 ** -30,
+                0.2 + 2.0 ** -40, 0.3 - 2.0 ** -41, 0.1 + 3 * 2.0 ** -45]
+    v[:len(specials)] = specials
+    X = v.reshape(16, 16, 4)
+    e = Engine("fp32")
+    slot = e.add_problem(pb)
+    e.set_field(slot, X)
+    back = e.field(slot, "X")
+    e.close()
+    hi = X.astype(np.float32)
+    q = np.spacing(np.maximum(np.abs(hi), np.float32(np.finfo(np.float32).tiny))).astype(np.float64) * 2.0 ** -15
+    err = np.abs(back - X)
+    assert np.all(err <= q / 2), (err / q).max()
+    # values with at most 39 significant bits come back exactly
+    exact = np.round(X / q) * q == X
+    assert np.array_equal(back[exact], X[exact])
+    assert exact.sum() >= 8  # 0, -0, 1, 1/2, 1/4, +-2^-20, 1 - 2^-30
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -396,7 +421,7 @@ def test_save_step_diagnostics_vs_oracle(dev):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_stencil_forms_bitwise(dev, precision):
-    """Every stencil tiling, single-pass or two-pass, gives the same bits: rows
+    """Every stencil tiling and form gives the same bits: rows
     longer than a tile (1D 600, 2D 96 = 64 + 32), ragged 3D tiles (20^3), with
     and without elasticity, in one batch."""
     from phasefield5d_amd.engine import Engine
