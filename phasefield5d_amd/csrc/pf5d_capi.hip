@@ -330,9 +330,13 @@ static PlaneKernels<T> plane_kernels_np() {
 }
 template <typename T, int NT>
 static PlaneKernels<T> plane_kernels_nt(int n, int parts) {
-  if (n == 128 && parts == 2) return plane_kernels_np<T, NT, 128, 2>();
-  if (n == 128 && parts == 4) return plane_kernels_np<T, NT, 128, 4>();
-  if (n == 128 && parts == 8) return plane_kernels_np<T, NT, 128, 8>();
+  // (128-point FP64 planes are not built for 1024 threads: held to 128 VGPRs,
+  // those kernels spill)
+  if constexpr (!(sizeof(T) == 8 && NT == 1024)) {
+    if (n == 128 && parts == 2) return plane_kernels_np<T, NT, 128, 2>();
+    if (n == 128 && parts == 4) return plane_kernels_np<T, NT, 128, 4>();
+    if (n == 128 && parts == 8) return plane_kernels_np<T, NT, 128, 8>();
+  }
   if constexpr (NT != 256) {
     if (n == 64 && parts == 2) return plane_kernels_np<T, NT, 64, 2>();
     if (n == 64 && parts == 4) return plane_kernels_np<T, NT, 64, 4>();

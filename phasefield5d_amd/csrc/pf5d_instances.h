@@ -87,9 +87,6 @@ namespace pf5d {
   PF5D_FFTP_INSTANCE(T, 512, 128, 2)  \
   PF5D_FFTP_INSTANCE(T, 512, 128, 4)  \
   PF5D_FFTP_INSTANCE(T, 512, 128, 8)  \
-  PF5D_FFTP_INSTANCE(T, 1024, 128, 2) \
-  PF5D_FFTP_INSTANCE(T, 1024, 128, 4) \
-  PF5D_FFTP_INSTANCE(T, 1024, 128, 8) \
   PF5D_FFTP_INSTANCE(T, 512, 64, 2)   \
   PF5D_FFTP_INSTANCE(T, 512, 64, 4)   \
   PF5D_FFTP_INSTANCE(T, 1024, 64, 2)  \
@@ -98,9 +95,14 @@ namespace pf5d {
   PF5D_FFTP_INSTANCE(T, 512, 96, 4)   \
   PF5D_FFTP_INSTANCE(T, 1024, 96, 2)  \
   PF5D_FFTP_INSTANCE(T, 1024, 96, 4)
-#define PF5D_FFTP_INSTANCES  \
-  PF5D_FFTP_INSTANCES_T(float) \
-  PF5D_FFTP_INSTANCES_T(double)
+// (128-point planes with 1024 threads in FP32 only: held to 128 VGPRs, the FP64
+// ones spill)
+#define PF5D_FFTP_INSTANCES                 \
+  PF5D_FFTP_INSTANCES_T(float)              \
+  PF5D_FFTP_INSTANCES_T(double)             \
+  PF5D_FFTP_INSTANCE(float, 1024, 128, 2)   \
+  PF5D_FFTP_INSTANCE(float, 1024, 128, 4)   \
+  PF5D_FFTP_INSTANCE(float, 1024, 128, 8)
 
 // the stencil forms the engine offers the autotuner
 #define PF5D_STENCIL_INSTANCES                    \
