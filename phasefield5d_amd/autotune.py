@@ -22,8 +22,13 @@ _lock = threading.Lock()
 
 
 def signature(engine):
+    """Cache key: precision, the multiset of shapes, and the library's table of
+    stencil variants (an entry names a variant by its index, which a rebuilt
+    library with other variants would give another meaning)."""
+    import hashlib
     shapes = sorted(tuple(pb.shape) for pb, _ in engine.slots.values())
-    return f"v6:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
+    table = hashlib.sha1(repr(engine.variants()).encode()).hexdigest()[:12]
+    return f"v7:{table}:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
 
 
 def _load():
