@@ -472,9 +472,9 @@ PF5D_DEV CellV<T> shfl_cell_from(const CellV<T>& v, int src) {
 // An item is a tile of rows (one per wavefront, at most NT/64) of up to 64*CPL
 // cells on axis 2, and a range [xa, xb) of planes on axis 0; lane i holds the
 // cells CPL*i .. CPL*i+CPL-1 of its wavefront's row.  Per plane:
-// - the x face below the plane is the x face above the previous plane (kept in
-//   registers), so one x face is evaluated per cell, plus one per cell for the
-//   range's first plane;
+// - the x face between the plane and the previous one in marching order (up
+//   or down the range, by build) was evaluated there and kept in registers, so
+//   one x face is evaluated per cell, plus one per cell for the first plane;
 // - the y face above a row uses the next row's cells, taken from LDS (the last
 //   row loads the row after the tile); the y face below a row is the face above
 //   the previous row, taken from LDS, and only the first row evaluates its own
@@ -556,14 +556,27 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
 #pragma unroll
       for (int j = 0; j < CPL; ++j) c[j] = load_cell32<T>(sd, r + kz[j]);
     };
+    // FP32 two-wave builds march down from the run's last plane: the planes the
+    // potentials pass wrote last are the first this pass reads (1 us at 128^3).
+    // Either way the x face carried from plane to plane (fxc) is the one between
+    // the plane and the previous one in marching order, evaluated lower cell
+    // first, so both directions give the same bits.  (FP64 and the three-wave
+    // builds march up: down, they spill.)
+    constexpr bool down = sizeof(T) == 4 && MINW < 3;
     CellV<T> me[CPL];
-    T fxd[CPL][4];
-    load_lane(item.xa, gy, me);
+    T fxc[CPL][4];
+    load_lane(down ? item.xb - 1 : item.xa, gy, me);
     if (hx) {
       CellV<T> pv[CPL];
-      load_lane(item.xa > 0 ? item.xa - 1 : n0 - 1, gy, pv);
+      if constexpr (down) {
+        load_lane(item.xb < n0 ? item.xb : 0, gy, pv);
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) face_ab<T>(pv[j], me[j], inv_dx, fxd[j]);
+        for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], pv[j], inv_dx, fxc[j]);
+      } else {
+        load_lane(item.xa > 0 ? item.xa - 1 : n0 - 1, gy, pv);
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) face_ab<T>(pv[j], me[j], inv_dx, fxc[j]);
+      }
     }
     // FP32: the tile's edge wavefronts load the halo row beyond it at the start
     // of each plane, so the load overlaps the exchange instead of stalling the
@@ -573,14 +586,19 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
     constexpr bool prefetch_halo = sizeof(T) == 4 && MINW < 3;
     const bool pre_up = prefetch_halo && hy && !ywrap && item.ty > 1 && w == item.ty - 1;
     const bool pre_dn = prefetch_halo && hy && !ywrap && item.ty > 1 && w == 0;
-    for (int x = item.xa; x < item.xb; ++x) {
+    for (int x = down ? item.xb - 1 : item.xa; down ? x >= item.xa : x < item.xb; x += down ? -1 : 1) {
       CellV<T> hal[CPL];
       if (pre_up)
         load_lane(x, gy + 1 < n1 ? gy + 1 : 0, hal);
       else if (pre_dn)
         load_lane(x, gy > 0 ? gy - 1 : n1 - 1, hal);
-      CellV<T> nx[CPL];
-      if (hx) load_lane(x + 1 < n0 ? x + 1 : 0, gy, nx);
+      CellV<T> nx[CPL];  // the next plane in marching order
+      if (hx) {
+        if constexpr (down)
+          load_lane(x > 0 ? x - 1 : n0 - 1, gy, nx);
+        else
+          load_lane(x + 1 < n0 ? x + 1 : 0, gy, nx);
+      }
       T fyu[CPL][4], fyd[CPL][4];
       if (hy) {
         put_cells<T, CPL, QC>(SC[w], lane, me);
@@ -696,11 +714,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
           face_ab<T>(dn, me[0], inv_dx, fzd[0]);
         }
       }
-      T fxu[CPL][4];
+      // the x face with the next plane (carried on to it); with the carried one,
+      // the faces above (fxu) and below (fxd) the plane
+      T fxn[CPL][4];
       if (hx) {
 #pragma unroll
-        for (int j = 0; j < CPL; ++j) face_ab<T>(me[j], nx[j], inv_dx, fxu[j]);
+        for (int j = 0; j < CPL; ++j) {
+          if constexpr (down)
+            face_ab<T>(nx[j], me[j], inv_dx, fxn[j]);
+          else
+            face_ab<T>(me[j], nx[j], inv_dx, fxn[j]);
+        }
       }
+      T(*fxu)[4] = down ? fxc : fxn;
+      T(*fxd)[4] = down ? fxn : fxc;
       const unsigned r = row_base(x, gy);
 #pragma unroll
       for (int j = 0; j < CPL; ++j) {
@@ -732,7 +759,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MINW, 8
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
 #pragma unroll
-          for (int a = 0; a < 4; ++a) fxd[j][a] = fxu[j][a];
+          for (int a = 0; a < 4; ++a) fxc[j][a] = fxn[j][a];
           me[j] = nx[j];
         }
       }
