@@ -3,7 +3,8 @@
 For an engine holding its simulations, every compiled K1 variant -- the
 two-pass forms (potentials tiles, with the flux pass's cells per lane and
 block size; the marching flux pass combined with x-chunk lengths for 3D) and
-the fused marching form (also with x-chunk lengths) -- is timed with HIP events on the engine
+the fused marching form (also with x-chunk lengths; only the marching forms
+when the simulations need more than one cache-sized group) -- is timed with HIP events on the engine
 stream (``Engine.time_kernel``); the fastest is kept.  Then the FFT: the line
 passes' batching (LDS per item, block size), the plane passes of 3D fields
 with 128 x 128 planes (parts per plane, block size), and the line batching
@@ -28,7 +29,7 @@ def signature(engine):
     import hashlib
     shapes = sorted(tuple(pb.shape) for pb, _ in engine.slots.values())
     table = hashlib.sha1(repr(engine.variants()).encode()).hexdigest()[:12]
-    return f"v7:{table}:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
+    return f"v8:{table}:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
 
 
 def _load():
@@ -66,10 +67,24 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
                            wave=ent.get("fft_wave", 0))
         return ent
     has3d = any(len(pb.shape) == 3 for pb, _ in engine.slots.values())
+    variants = engine.variants()
+    # Only the marching forms (fcpl > 0) step simulations in cache-sized groups.
+    # When the mix needs more than one group, each group's fields then stay in
+    # the last-level cache through the step's passes, which timing the stencil
+    # stage by itself does not see (64-run mix: 3.2 ms per step with a marching
+    # form, 3.8 with the per-plane form whose stencil stage alone is faster), so
+    # the candidates are the marching forms.
+    march0 = next((v for v, t in enumerate(variants) if t[3] > 0), None)
+    grouped = False
+    if march0 is not None:
+        engine.set_tiling(march0, chunks[0])
+        grouped = engine.num_groups() > 1
     results = []
-    for v, (by, bz, nt, fcpl, fnt) in enumerate(engine.variants()):
+    for v, (by, bz, nt, fcpl, fnt) in enumerate(variants):
         # the x-chunk length applies to the marching passes (fcpl > 0)
         marching = fcpl > 0
+        if grouped and not marching:
+            continue
         for cx in (chunks if has3d and marching else chunks[:1]):
             try:
                 engine.set_tiling(v, cx)
