@@ -216,7 +216,8 @@ int pf5d_engine_num_groups(pf5d_engine* e, int* ngroups);
 
 /* average device time of one launch of a stage (HIP events on the engine stream):
  * which = 0 stencil stage K1 (both of its passes), 1 the FFT passes, 2 dt controller, 3
- * update kernel in its statistics-only mode, 4 / 5 the potentials / flux pass of K1.  The timed launches rewrite the dX/dt and psi fields
+ * update kernel in its statistics-only mode, 4 / 5 the potentials / flux pass of K1, 6
+ * the FFT passes, K1 and the statistics-only update in sequence (K1 amid a step's traffic).  The timed launches rewrite the dX/dt and psi fields
  * from the current state; the call then restores what the next step reads (the
  * max|dX/dt| accumulator, the FP64 statistics), so the trajectory is unchanged. */
 int pf5d_engine_time_kernel(pf5d_engine* e, int which, int reps, double* avg_ms);

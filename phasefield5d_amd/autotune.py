@@ -5,7 +5,8 @@ two-pass forms (potentials tiles, with the flux pass's cells per lane and
 block size; the marching flux pass combined with x-chunk lengths for 3D) and
 the fused marching form (also with x-chunk lengths; only the marching forms
 when the simulations need more than one cache-sized group) -- is timed with HIP events on the engine
-stream (``Engine.time_kernel``); the fastest is kept.  Then the FFT: the line
+stream (``Engine.time_kernel``, between the FFT passes and an update launch as
+in a step); the fastest is kept.  Then the FFT: the line
 passes' batching (LDS per item, block size), the plane passes of 3D fields
 with 128 x 128 planes (parts per plane, block size), and the line batching
 again for the x pass the planes leave.  Results are cached per (precision,
@@ -29,7 +30,7 @@ def signature(engine):
     import hashlib
     shapes = sorted(tuple(pb.shape) for pb, _ in engine.slots.values())
     table = hashlib.sha1(repr(engine.variants()).encode()).hexdigest()[:12]
-    return f"v8:{table}:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
+    return f"v9:{table}:{engine.precision}:" + ";".join("x".join(map(str, s)) for s in shapes)
 
 
 def _load():
@@ -88,7 +89,10 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
         for cx in (chunks if has3d and marching else chunks[:1]):
             try:
                 engine.set_tiling(v, cx)
-                ms = engine.time_kernel("k1", reps=reps)
+                # the stencil stage between the FFT passes and the update, as in a
+                # step (what the other passes leave in the caches decides part of
+                # its time); those passes are the same for every candidate
+                ms = engine.time_kernel("step_like", reps=reps)
             except (ValueError, RuntimeError) as e:  # tiling does not fit this device
                 if verbose:
                     print(f"  variant {v} ({by}x{bz}/{nt}) cx={cx}: skipped ({e})")
@@ -100,7 +104,8 @@ def tune(engine, chunks=(8, 16, 32), reps=10, use_cache=True, verbose=False, fft
         raise RuntimeError("no stencil tiling could be launched")
     ms, v, cx = min(results)
     engine.set_tiling(v, cx)
-    entry = {"variant": v, "cx": cx, "k1_ms": ms, "tile": list(engine.variants()[v])}
+    entry = {"variant": v, "cx": cx, "step_like_ms": ms, "k1_ms": engine.time_kernel("k1", reps=reps),
+             "tile": list(engine.variants()[v])}
     # FFT passes: lines per block (LDS budget) and block size of the line passes,
     # then the plane passes (parts per plane and block size) of 3D fields with
     # 128 x 128 planes, then the line passes again for what the planes leave

@@ -1821,9 +1821,11 @@ struct Engine : EngineBase {
   }
   // average duration of one launch of a single stage, timed with HIP events on
   // the engine stream: 0 = K1 stencil, 1 = FFT stages (all five), 2 = DT, 3 = K2 prime,
-  // 4 / 5 = potentials / flux pass of the two-pass stencil
+  // 4 / 5 = potentials / flux pass of the two-pass stencil; 6 = the FFT passes, K1
+  // and a K2 prime in sequence (the stencil stage between the passes a step runs
+  // around it, whose traffic leaves the caches as a step does)
   double time_kernel(int which, int reps) override {
-    require(reps > 0 && which >= 0 && which <= 5, "time_kernel: bad arguments");
+    require(reps > 0 && which >= 0 && which <= 6, "time_kernel: bad arguments");
     if (dirty) rebuild();
     const K1Variant<T>& V = variants[variant];
     hipEvent_t a, b;
@@ -1836,8 +1838,13 @@ struct Engine : EngineBase {
       else if (which == 3) launch_k2(1);
       else if (which == 4)
         hipLaunchKernelGGL(pot(V), dim3(k1_grid), dim3(V.nt), 0, stream, d_sims, d_state, d_k1, nk1);
-      else
+      else if (which == 5)
         launch_flux(V);
+      else {
+        fft.run(stream, d_state);
+        launch_k1();
+        launch_k2(1);
+      }
     };
     one();
     HIPCHK(hipEventRecord(a, stream));

@@ -175,7 +175,7 @@ class Engine:
     def time_kernel(self, which="k1", reps=20):
         """Average milliseconds of one launch of a stage (HIP events on the engine stream)."""
         ms = C.c_double()
-        w = {"k1": 0, "fft": 1, "dt": 2, "k2prime": 3, "k1_potentials": 4, "k1_flux": 5}[which]
+        w = {"k1": 0, "fft": 1, "dt": 2, "k2prime": 3, "k1_potentials": 4, "k1_flux": 5, "step_like": 6}[which]
         _lib.check(_lib.lib().pf5d_engine_time_kernel(self.h, w, int(reps), C.byref(ms)))
         return ms.value
 

@@ -65,10 +65,10 @@ def test_kernel_timing_leaves_trajectory_unchanged(dev, precision):
     X_ref, st_ref, _ = _run(pb, precision, lambda e: e.step(n))
 
     def timed(e):
-        for which in ("k1", "k1_potentials", "k1_flux", "dt", "fft", "k2prime"):
+        for which in ("k1", "k1_potentials", "k1_flux", "dt", "fft", "k2prime", "step_like"):
             e.time_kernel(which, reps=3)
         e.step(3)
-        for which in ("k1", "dt", "k1_flux"):
+        for which in ("k1", "dt", "k1_flux", "step_like"):
             e.time_kernel(which, reps=2)
         e.step(n - 3)
 
