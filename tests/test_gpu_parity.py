@@ -342,6 +342,25 @@ def test_fp32_state_remainder_roundtrip(dev):
     assert exact.sum() >= 8  # 0, -0, 1, 1/2, 1/4, +-2^-20, 1 - 2^-30
 
 
+@pytest.mark.parametrize("precision,side", [("fp64", 520), ("fp32", 650)])
+def test_engine_rejects_fields_past_32bit_offsets(dev, precision, side):
+    """A simulation whose (ncell, 4) fields pass 4 GiB (the stencil kernels' 32-bit
+    offsets: 2^27 cells in FP64, 2^28 in FP32) is refused with ValueError before
+    anything is allocated; the engine keeps working."""
+    import dataclasses
+    from phasefield5d_amd.engine import Engine
+    base = problem(3, 16, elastic=False)
+    big = dataclasses.replace(base, shape=(side, side, side), X_init=None, Bhat=None,
+                              extra={"device_init": (1, 1e-3)})
+    e = Engine(precision)
+    with pytest.raises(ValueError, match="field too large"):
+        e.add_problem(big)
+    s = e.add_problem(base)
+    e.step(2)
+    assert e.state(s)["steps"] == 2
+    e.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_retiling_midrun_keeps_trajectory(dev, precision):
     """Changing the stencil tiling / FFT batching between steps rebuilds the work lists;
