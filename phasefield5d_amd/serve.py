@@ -255,3 +255,43 @@ def summarize(done, wall):
             "audit_max_err": max(errs) if errs else None,
             "failed": sum(1 for r in done if r.status & ST_HARD_OOB),
             "completed": sum(1 for r in done if r.status & ST_DONE)}
+
+
+def main(argv=None):
+    """Serve a Poisson stream of mixed requests for a given window and report
+    latency, throughput and the FP64 audit of a sample as one JSON object:
+
+        python -m phasefield5d_amd.serve --rate 4 --seconds 600 --max-steps 300 --json out.json
+
+    Prints a line per completed request and per audit while it runs."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="continuous-batching serving run (one GPU, FP32)")
+    ap.add_argument("--rate", type=float, default=4.0, help="mean arrivals per second")
+    ap.add_argument("--seconds", type=float, default=20.0, help="arrival window")
+    ap.add_argument("--max-steps", type=int, default=300, help="CFL-limited steps per request at most")
+    ap.add_argument("--audit-frac", type=float, default=0.05, help="fraction of requests re-run alone in FP64")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--json", default=None, help="write the summary here as well")
+    a = ap.parse_args(argv)
+    reqs = poisson_requests(a.rate, a.seconds, seed=a.seed, max_steps=a.max_steps, audit_frac=a.audit_frac)
+    print(f"{len(reqs)} requests over {a.seconds:g} s", flush=True)
+    srv = Server("fp32", poll_steps=25, setup_workers=6)
+    done = srv.run(reqs, max_wall=a.seconds * 6 + 60, verbose=True)
+    srv.close()
+    for r in done:
+        if r.audit:
+            err = audit(r)
+            print(f"  audit {r.rid} {r.problem.shape}: max |X - X_fp64| {err:.3e}", flush=True)
+    out = summarize(done, srv.wall)
+    out.update({"arrival_rate_per_s": a.rate, "arrival_window_s": a.seconds, "submitted": len(reqs),
+                "max_steps_per_request": a.max_steps, "gpu_busy_s": srv.busy,
+                "audited_fraction": sum(1 for r in done if r.audit) / max(1, len(done))})
+    print(json.dumps(out), flush=True)
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
