@@ -139,8 +139,8 @@ PF5D_DEV auto vget(const V& v, int i) {
 }
 
 // The 16 corner rows are gathered in batches of one 16-byte column per corner
-// (16 independent loads in flight per batch, one wait per batch); within each
-// property the sum still runs c0000 .. c1111 left to right.  ``base`` points at
+// (16 independent loads in flight per batch from LDS, 8 from HBM; one wait per
+// batch); within each property the sum still runs c0000 .. c1111 left to right.  ``base`` points at
 // the row of the lower corner; n1, n2, n3 are the element strides of axes 2, 1, 0
 // (the stride of axis 3 is one row).  The rows may live in the HBM table or in
 // an LDS copy of a box of nodes: the arithmetic is the same.
