@@ -752,18 +752,18 @@ __global__ void __launch_bounds__(INIT_NT) k_init_field(T* __restrict__ X, Rem4*
     }
     m /= 5.0;
     V4<T> hi;
-    Rem4 lo;
+    int lo[4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const double x = X0[a] + f * (n[a + 1] - m);
       hi.v[a] = (T)x;
-      lo.v[a] = rem_encode(x, (float)hi.v[a]);
+      lo[a] = rem_encode(x, (float)hi.v[a]);
       const double t = sum[a] + x;
       cmp[a] += (fabs(sum[a]) >= fabs(x)) ? ((sum[a] - t) + x) : ((x - t) + sum[a]);
       sum[a] = t;
     }
     st4g<T>(X + 4 * c, hi);
-    if (Xlo) gst<Rem4>(Xlo + c, lo);
+    if (Xlo) st_rem(Xlo + c, rem_pack(lo));
   }
   __shared__ double red[INIT_NT / 64][8];
   double v[8] = {sum[0], cmp[0], sum[1], cmp[1], sum[2], cmp[2], sum[3], cmp[3]};
@@ -1030,9 +1030,9 @@ struct Engine : EngineBase {
     }
   };
   std::vector<std::unique_ptr<Group>> groups;
-  // per-step working set, bytes per cell: X (+ 16-bit remainder), dX, mu, Mob, s,
+  // per-step working set, bytes per cell: X (+ 24-bit remainders), dX, mu, Mob, s,
   // psi and the spectrum; a group holds at most GROUP_BYTES of it
-  static constexpr size_t STEP_BYTES = sizeof(T) == 4 ? 92 : 170;
+  static constexpr size_t STEP_BYTES = sizeof(T) == 4 ? 96 : 170;
   static constexpr size_t GROUP_BYTES = size_t(192) << 20;
 
   Engine(hipStream_t s) {
@@ -1234,7 +1234,7 @@ struct Engine : EngineBase {
     *slot = id;
     dirty = true;
   }
-  // FP32 mode keeps X as the nearest FP32 value plus a 16-bit remainder
+  // FP32 mode keeps X as the nearest FP32 value plus a 24-bit remainder
   void upload_X(Sim& s, const double* Xh) {
     // FP64 mean of the field (pairwise over cells is unnecessary at this size: Neumaier)
     for (int a = 0; a < 4; ++a) {
@@ -1247,13 +1247,17 @@ struct Engine : EngineBase {
       s.mean0[a] = (sum + c) / (double)s.ncell;
     }
     std::vector<T> h(s.ncell * 4);
-    std::vector<short> lo(s.Xlo ? s.ncell * 4 : 0);
-    for (long i = 0; i < s.ncell * 4; ++i) {
-      h[i] = (T)Xh[i];
-      if (s.Xlo) lo[i] = rem_encode(Xh[i], (float)h[i]);
+    std::vector<Rem4> lo(s.Xlo ? s.ncell : 0);
+    for (long c = 0; c < s.ncell; ++c) {
+      int v[4];
+      for (int a = 0; a < 4; ++a) {
+        h[4 * c + a] = (T)Xh[4 * c + a];
+        v[a] = rem_encode(Xh[4 * c + a], (float)h[4 * c + a]);
+      }
+      if (s.Xlo) lo[c] = rem_pack(v);
     }
     HIPCHK(hipMemcpy(s.X, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-    if (s.Xlo) HIPCHK(hipMemcpy(s.Xlo, lo.data(), lo.size() * sizeof(short), hipMemcpyHostToDevice));
+    if (s.Xlo) HIPCHK(hipMemcpy(s.Xlo, lo.data(), lo.size() * sizeof(Rem4), hipMemcpyHostToDevice));
   }
   Sim& at(int slot) {
     require(slot >= 0 && slot < (int)sims.size() && sims[slot].used, "invalid slot " + std::to_string(slot));
@@ -1755,11 +1759,19 @@ struct Engine : EngineBase {
     require(src != nullptr, "field not present");
     const long n = (which == 2) ? s.ncell : s.ncell * 4;
     std::vector<T> h(n);
-    std::vector<short> lo((which == 0 && s.Xlo) ? n : 0);
+    std::vector<Rem4> lo((which == 0 && s.Xlo) ? s.ncell : 0);
     HIPCHK(hipMemcpyAsync(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
-    if (!lo.empty()) HIPCHK(hipMemcpyAsync(lo.data(), s.Xlo, n * sizeof(short), hipMemcpyDeviceToHost, stream));
+    if (!lo.empty()) HIPCHK(hipMemcpyAsync(lo.data(), s.Xlo, lo.size() * sizeof(Rem4), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    for (long i = 0; i < n; ++i) out[i] = lo.empty() ? (double)h[i] : rem_decode((float)h[i], lo[i]);
+    if (lo.empty()) {
+      for (long i = 0; i < n; ++i) out[i] = (double)h[i];
+    } else {
+      for (long c = 0; c < s.ncell; ++c) {
+        int v[4];
+        rem_unpack(lo[c], v);
+        for (int a = 0; a < 4; ++a) out[4 * c + a] = rem_decode((float)h[4 * c + a], v[a]);
+      }
+    }
   }
   void set_field(int slot, const double* X) override {
     Sim& s = at(slot);

@@ -116,15 +116,16 @@ PF5D_DEV void gst_field_o(void* base, unsigned off, const V& v) {
   gst_field<T, V>(static_cast<char*>(base) + off, v);
 }
 
-// FP32 state remainder in 16 bits.  The FP64 state x is held as the nearest
-// float hi and lo = rint((x - hi) / q), q = ulp(hi) / 2^15 (for zero and
+// FP32 state remainder in 24 bits.  The FP64 state x is held as the nearest
+// float hi and lo = rint((x - hi) / q), q = ulp(hi) / 2^23 (for zero and
 // subnormal hi the ulp of the smallest normal float): |x - hi| <= ulp(hi) / 2,
-// so |lo| <= 2^14, and hi + lo q is within q / 2 of x -- a 39-bit significand
-// whose exact FP64 sum needs no rounding.  q is a power of two (2^(b - 165) for
+// so |lo| <= 2^22, and hi + lo q is within q / 2 of x -- a 47-bit significand
+// whose exact FP64 sum needs no rounding.  q is a power of two (2^(b - 173) for
 // the biased exponent b of hi), so every scaling is exact.  Infinite or NaN hi
-// keeps lo = 0.
-struct alignas(8) Rem4 {
-  short v[4];
+// keeps lo = 0.  A cell's four remainders pack into three 32-bit words (12 B
+// against 16 for four float remainders).
+struct alignas(4) Rem4 {
+  unsigned w[3];
 };
 PF5D_HD int rem_exp(float hi) {
   unsigned bits;
@@ -132,12 +133,43 @@ PF5D_HD int rem_exp(float hi) {
   const int b = (int)((bits >> 23) & 0xFFu);
   return b < 1 ? 1 : b;
 }
-PF5D_HD short rem_encode(double x, float hi) {
+PF5D_HD int rem_encode(double x, float hi) {
   const int b = rem_exp(hi);
   if (b == 255) return 0;
-  return (short)(int)__builtin_rint((x - (double)hi) * __builtin_ldexp(1.0, 165 - b));
+  return (int)__builtin_rint((x - (double)hi) * __builtin_ldexp(1.0, 173 - b));
 }
-PF5D_HD double rem_decode(float hi, short lo) { return (double)hi + (double)lo * __builtin_ldexp(1.0, rem_exp(hi) - 165); }
+PF5D_HD double rem_decode(float hi, int lo) { return (double)hi + (double)lo * __builtin_ldexp(1.0, rem_exp(hi) - 173); }
+// four 24-bit two's-complement values: v0 in w0[0:24), v1 in w0[24:32) and
+// w1[0:16), v2 in w1[16:32) and w2[0:8), v3 in w2[8:32)
+PF5D_HD Rem4 rem_pack(const int v[4]) {
+  Rem4 r;
+  r.w[0] = ((unsigned)v[0] & 0xFFFFFFu) | ((unsigned)v[1] << 24);
+  r.w[1] = (((unsigned)v[1] >> 8) & 0xFFFFu) | ((unsigned)v[2] << 16);
+  r.w[2] = (((unsigned)v[2] >> 16) & 0xFFu) | ((unsigned)v[3] << 8);
+  return r;
+}
+PF5D_HD void rem_unpack(const Rem4& r, int v[4]) {
+  v[0] = (int)(r.w[0] << 8) >> 8;
+  v[1] = (int)(((r.w[0] >> 24) | (r.w[1] << 8)) << 8) >> 8;
+  v[2] = (int)(((r.w[1] >> 16) | (r.w[2] << 16)) << 8) >> 8;
+  v[3] = (int)r.w[2] >> 8;
+}
+// a cell's remainders in HBM, as three 32-bit accesses (a 12-byte vector type
+// would be padded to 16 bytes and read past the last cell)
+PF5D_DEV Rem4 ld_rem(const Rem4* p) {
+  const unsigned* q = reinterpret_cast<const unsigned*>(p);
+  Rem4 r;
+  r.w[0] = gld<unsigned>(q);
+  r.w[1] = gld<unsigned>(q + 1);
+  r.w[2] = gld<unsigned>(q + 2);
+  return r;
+}
+PF5D_DEV void st_rem(Rem4* p, const Rem4& r) {
+  unsigned* q = reinterpret_cast<unsigned*>(p);
+  gst<unsigned>(q, r.w[0]);
+  gst<unsigned>(q + 1, r.w[1]);
+  gst<unsigned>(q + 2, r.w[2]);
+}
 
 template <typename T>
 PF5D_DEV V4<T> ld4g(const T* p) { return gld<V4<T>>(p); }

@@ -40,7 +40,7 @@ struct SimDev {
   int fresh;          // the field was just set: a prime launch writes its misfit s
   TableView<T> tab;
   T* X;
-  Rem4* Xlo;          // FP32 mode: 16-bit rounding remainder of each value (rem_decode; null in FP64 mode)
+  Rem4* Xlo;          // FP32 mode: 24-bit rounding remainders of a cell's values (rem_decode; null in FP64 mode)
   T* dX;
   const T* psi;
   T* s;
@@ -250,9 +250,10 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
       V4<T> x = ld4g<T>(sd.X + 4 * c);
       double xd[4];  // the state in FP64 (FP32 mode: X with its remainder)
       if (sd.Xlo) {
-        const Rem4 lo = gld<Rem4>(sd.Xlo + c);
+        int lo[4];
+        rem_unpack(ld_rem(sd.Xlo + c), lo);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) xd[a] = rem_decode((float)x.v[a], lo.v[a]);
+        for (int a = 0; a < 4; ++a) xd[a] = rem_decode((float)x.v[a], lo[a]);
       } else {
 #pragma unroll
         for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
@@ -273,7 +274,7 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
           for (int a = 0; a < 4; ++a) xd[a] = (double)x.v[a];
           st4g<T>(sd.X + 4 * c, x);
         } else {
-          // FP32 state with a 16-bit remainder: update and clamp in FP64, then split
+          // FP32 state with a 24-bit remainder: update and clamp in FP64, then split
           // into the nearest FP32 value and what it leaves out, so increments below
           // half an FP32 ulp accumulate instead of being rounded away
 #pragma unroll
@@ -284,14 +285,14 @@ __global__ void __launch_bounds__(NT) k_update(const SimDev<T>* __restrict__ sim
 #pragma unroll
             for (int a = 0; a < 4; ++a) xd[a] = xd[a] * inv;
           }
-          Rem4 lo;
+          int lo[4];
 #pragma unroll
           for (int a = 0; a < 4; ++a) {
             x.v[a] = (T)xd[a];
-            lo.v[a] = rem_encode(xd[a], (float)x.v[a]);
+            lo[a] = rem_encode(xd[a], (float)x.v[a]);
           }
           st4g<T>(sd.X + 4 * c, x);
-          gst<Rem4>(sd.Xlo + c, lo);
+          st_rem(sd.Xlo + c, rem_pack(lo));
         }
       }
       // the misfit s: from the updated state in a step; a prime launch writes it

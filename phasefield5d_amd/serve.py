@@ -57,7 +57,7 @@ def footprint_bytes(pb, precision):
     """Device bytes a request holds in an engine (fields, remainder, spectrum, kernel)."""
     w = 4 if precision == "fp32" else 8
     n = pb.ncell
-    b = n * 4 * w * 2 + (n * 8 if precision == "fp32" else 0)  # X, dX (+ 16-bit remainder in FP32)
+    b = n * 4 * w * 2 + (n * 12 if precision == "fp32" else 0)  # X, dX (+ 24-bit remainders in FP32)
     b += n * 9 * w                                         # mu, Mob of the two-pass stencil
     if pb.Bhat is not None:
         kshape = list(pb.Bhat.shape)

@@ -33,11 +33,9 @@ def test_groups_bitwise_equal_alone(dev, precision):
     from phasefield5d_amd.engine import Engine
     from phasefield5d_amd.problem import build_problem
     from phasefield5d_amd.workloads import workload_config
-    # two 128^3 fields (no two fit one group: in FP32 each working set is most of
-    # the group budget, in FP64 more than it), then ten small 2D fields, which
-    # join the second one's group in FP32 and form their own in FP64 (either way
-    # a group of more than eight simulations, whose dt controllers run as their
-    # own launch)
+    # two 128^3 fields (each one group: its working set alone is about the group
+    # budget), then ten small 2D fields (one group of more than eight simulations,
+    # whose dt controllers run as their own launch)
     pbs = [build_problem(workload_config("3d_128", seed=s)) for s in (0, 1)]
     for k in range(10):
         p = problem(2, 32, seed=40 + k)
@@ -46,7 +44,7 @@ def test_groups_bitwise_equal_alone(dev, precision):
     steps = 19  # one graph of 16 steps per group, then 3 single steps
     e = Engine(precision)
     slots = [e.add_problem(p) for p in pbs]
-    assert e.num_groups() == (2 if precision == "fp32" else 3)
+    assert e.num_groups() == 3
     e.step(steps)
     got = [(e.field(s, "X"), e.state(s)) for s in slots]
     e.close()
@@ -72,11 +70,11 @@ def test_groups_rebuilt_after_removal(dev):
         pbs.append(p)
     e = Engine("fp32")
     slots = [e.add_problem(p) for p in pbs]
-    assert e.num_groups() == 3  # one per 128^3 field, the small ones with the last
+    assert e.num_groups() == 4  # one per 128^3 field, one for the three small ones
     e.step(17)
     e.remove(slots[1])  # a 128^3 group disappears
     e.remove(slots[4])  # a small one from the last group
-    assert e.num_groups() == 2
+    assert e.num_groups() == 3
     e.step(6)
     keep = [0, 2, 3, 5]
     got = {i: (e.field(slots[i], "X"), e.state(slots[i])) for i in keep}

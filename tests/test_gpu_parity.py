@@ -313,10 +313,10 @@ def test_fp32_engine_error(dev):
 
 
 def test_fp32_state_remainder_roundtrip(dev):
-    """The FP32 engine holds a field as the nearest float plus a 16-bit remainder
-    in units of 2^-15 of the float spacing: set_field / field round-trips every
-    value to within 2^-16 of that spacing (exactly where the value has at most
-    39 significant bits), across magnitudes, signs, zeros, subnormal floats and
+    """The FP32 engine holds a field as the nearest float plus a 24-bit remainder
+    in units of 2^-23 of the float spacing: set_field / field round-trips every
+    value to within 2^-24 of that spacing (exactly where the value has at most
+    47 significant bits), across magnitudes, signs, zeros, subnormal floats and
     values on either side of powers of two."""
     from phasefield5d_amd.engine import Engine
     pb = problem(2, 16, elastic=False)
@@ -333,7 +333,7 @@ def test_fp32_state_remainder_roundtrip(dev):
     back = e.field(slot, "X")
     e.close()
     hi = X.astype(np.float32)
-    q = np.spacing(np.maximum(np.abs(hi), np.float32(np.finfo(np.float32).tiny))).astype(np.float64) * 2.0 ** -15
+    q = np.spacing(np.maximum(np.abs(hi), np.float32(np.finfo(np.float32).tiny))).astype(np.float64) * 2.0 ** -23
     err = np.abs(back - X)
     assert np.all(err <= q / 2), (err / q).max()
     # values with at most 39 significant bits come back exactly

@@ -110,7 +110,7 @@ def _splitmix_uniform(seed, i):
 
 def test_device_drawn_initial_field():
     """A field drawn on the device: bit-equal to a restatement of the generator in
-    FP64; the FP32 engine holds the same field as value + 16-bit remainder; the engine's
+    FP64; the FP32 engine holds the same field as value + 24-bit remainder; the engine's
     FP64 mass equals the field's; a served request with a device-drawn field equals
     the same request run alone."""
     import dataclasses
@@ -129,10 +129,10 @@ def test_device_drawn_initial_field():
         m = ((((n[0] + n[1]) + n[2]) + n[3]) + n[4]) / 5.0
         ref[c] = [pb.X0[a] + 1e-3 * (n[a + 1] - m) for a in range(4)]
     assert np.array_equal(X64.reshape(-1, 4), ref)
-    # the FP32 engine's value + 16-bit remainder holds the field to half a
-    # quantum, 2^-16 of the float spacing at the value
+    # the FP32 engine's value + 24-bit remainder holds the field to half a
+    # quantum, 2^-24 of the float spacing at the value
     hi = X64.astype(np.float32)
-    assert np.all(np.abs(X32 - X64) <= np.spacing(hi).astype(np.float64) * 2.0**-16)
+    assert np.all(np.abs(X32 - X64) <= np.spacing(hi).astype(np.float64) * 2.0**-24)
     assert np.allclose(mass64, X64.reshape(-1, 4).sum(axis=0), rtol=1e-14, atol=0)
     # served with the field drawn at admission == alone
     from phasefield5d_amd.serve import Request, Server
