@@ -971,7 +971,8 @@ struct Engine : EngineBase {
   // x planes per potentials item of the marching two-pass form (3D): the item's
   // node box, tile set-up and barriers serve that many cells per thread.  At
   // 128^3, per-plane items take KP 37.5 / 69 us (FP32 / FP64); 2, 4 and 8 planes
-  // take 31.8, 28.9, 27.1 us in FP32 and 62, 58, 63 us in FP64
+  // take 31.8, 28.9, 27.1 us in FP32 and 62, 58, 63 us in FP64 (16 planes in
+  // FP32, whose 512 items leave two blocks per CU: 33 us)
   int kp_planes = sizeof(T) == 4 ? 8 : 4;
   // device arrays
   SimDev<T>* d_sims = nullptr;
